@@ -1,0 +1,147 @@
+/*
+ * phrike_hip.h -- C ABI of the MI355X (gfx950) backend for the pseudo-spectral
+ * Euler right-hand side and RK4 step.
+ *
+ * The reference has no FFI: its backend switch is the string `backend`
+ * threaded from phrike.run_simulation to Grid*(backend=...)
+ * (phrike/__init__.py:14-56, phrike/grid.py:1424-1426, 924) and the solver is
+ * plain array algebra on whatever the grid returns (phrike/solver.py:1177-1337).
+ * Each entry point below replaces one reference routine, cited per function;
+ * the Python host layer (phrike_amd) binds them with ctypes.
+ *
+ * Conventions
+ *   - All array arguments are device pointers (HIP device memory) in the
+ *     reference layout: (ncomp, Nz, Ny, Nx) for 3D, (batch, ncomp, Ny, Nx)
+ *     for batched 2D, x fastest, C order, element type float64 or float32 as
+ *     chosen at plan creation (PH_F64 / PH_F32).
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Every call is
+ *     asynchronous on that stream; nothing synchronises the device.
+ *   - `ws` is caller-owned device scratch of at least *_workspace_bytes();
+ *     concurrent calls need distinct workspaces.
+ *   - Return value 0 = success, negative = error; ph_last_error() returns a
+ *     thread-local message for the last failure.
+ *   - Grid sizes are powers of two in [8, 1024] along every axis.
+ */
+#ifndef PHRIKE_HIP_H
+#define PHRIKE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PH_F64 0
+#define PH_F32 1
+
+#define PH_OK 0
+#define PH_ERR_ARG (-2)
+#define PH_ERR_HIP (-5)
+#define PH_ERR_INTERNAL (-9)
+
+typedef struct ph_plan3d_s* ph_plan3d_t;
+typedef struct ph_plan2d_s* ph_plan2d_t;
+
+/* Library ABI version (no reference counterpart). */
+int ph_abi_version(void);
+
+/* Message for the last failed call on this thread.  Replaces the reference's
+ * Python exceptions (ValueError/RuntimeError, phrike/grid.py:66-84, 308). */
+const char* ph_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * 3D periodic grid.  Replaces Grid3D.__post_init__ (phrike/grid.py:1378-1415:
+ * wavenumbers, 2/3 mask, exponential filter tables).  filter_p/filter_alpha
+ * follow integration.spectral_filter {enabled, p, alpha}
+ * (phrike/problems/base.py:103-105; defaults p=8, alpha=36 at grid.py:1411).
+ * ------------------------------------------------------------------------ */
+int ph_plan3d_create(ph_plan3d_t* plan, int nx, int ny, int nz, double lx, double ly, double lz,
+                     int dtype, int dealias, int filter_enabled, int filter_p, double filter_alpha,
+                     double gamma);
+int ph_plan3d_destroy(ph_plan3d_t plan);
+int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes);
+
+/* rhs = -(dx Fx + dy Fy + dz Fz) from U (5 fields).
+ * Replaces _compute_rhs_3d (phrike/solver.py:1177-1184) with
+ * EulerEquations3D.flux (phrike/equations.py:582-607) and Grid3D.dx1/dy1/dz1
+ * (phrike/grid.py:1705-1721). */
+int ph_rhs3d(ph_plan3d_t plan, const void* U, void* rhs, void* ws, void* stream);
+
+/* out = d/d(axis) f for nfields (1..5) fields; axis 0 = x, 1 = y, 2 = z;
+ * axis -1 = Grid3D.apply_spectral_filter.  Replaces Grid3D.dx1/dy1/dz1 and
+ * apply_spectral_filter (phrike/grid.py:1705-1726).  in == out is allowed. */
+int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out, void* ws,
+               void* stream);
+
+/* One RK4 step followed by the post-step spectral filter:
+ * out = filter(RK4(U, dt)).  dt is taken from dt_dev[0] when dt_dev is not
+ * NULL (device FP64), else from `dt`.  U and out must not overlap.
+ * Replaces SpectralSolver3D.step/_fixed_step/_rk4_step_3d/
+ * _apply_physical_filters_3d (phrike/solver.py:1220-1234, 1313-1337). */
+int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev,
+              void* ws, void* stream);
+
+/* max over cells of max(|ux|,|uy|,|uz|) + sqrt(gamma p / rho) -> max_dev[0]
+ * (FP64), and when dt_dev is not NULL dt_dev[0] = cfl*min(dx,dy,dz)/max (1e-6
+ * if max <= 0).  Replaces EulerEquations3D.max_wave_speed
+ * (phrike/equations.py:609-620, 199-211) and SpectralSolver3D.compute_dt
+ * (phrike/solver.py:1307-1311).  Deterministic two-level reduction. */
+int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double* dt_dev,
+                        double cfl, void* ws, void* stream);
+
+/* FP64 sums over cells of rho, rho*ux, rho*uy, rho*uz, p/(gamma-1)+rho|u|^2/2
+ * -> out5_dev.  Replaces the record() sums of SpectralSolver3D.run
+ * (phrike/solver.py:1439-1468) and EulerEquations3D.conserved_quantities
+ * (phrike/equations.py:622-637).  Deterministic. */
+int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void* ws, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batched 2D Fourier x Fourier grid: `batch` independent runs of one shape,
+ * state (batch, 4, ny, nx).  Replaces Grid2D Fourier mode
+ * (phrike/grid.py:846-911, 1157-1287) for every run of a batch.
+ * ------------------------------------------------------------------------ */
+int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, int dtype,
+                     int dealias, int filter_enabled, int filter_p, double filter_alpha,
+                     double gamma, int batch);
+int ph_plan2d_destroy(ph_plan2d_t plan);
+int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes);
+
+/* rhs = -(dx Fx + dy Fy) for the first nbatch runs.  Replaces
+ * _compute_rhs_2d (phrike/solver.py:428-443). */
+int ph_rhs2d(ph_plan2d_t plan, int nbatch, const void* U, void* rhs, void* ws, void* stream);
+
+/* 2D derivative / filter of the 4 fields of each run (axis 0, 1, or -1 for
+ * the filter).  Replaces Grid2D.dx1/dy1/apply_spectral_filter
+ * (phrike/grid.py:1226-1287). */
+int ph_deriv2d(ph_plan2d_t plan, int nbatch, int axis, const void* f, void* out, void* ws,
+               void* stream);
+
+/* One RK4 step with the positivity clamp after every stage and at the end,
+ * then the filter if apply_filter != 0.  Per-run dt from dt_dev[run] (FP64)
+ * when dt_dev is not NULL, else `dt` for all runs.  Runs with
+ * active_dev[run] == 0 are left untouched in `out` (active_dev may be NULL).
+ * U and out must not overlap.  Replaces SpectralSolver2D._fixed_step,
+ * _rk4_step_2d, _positivity_clamp_2d, _apply_physical_filters_2d
+ * (phrike/solver.py:467-524, 639-648). */
+int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
+              const double* dt_dev, const int* active_dev, int apply_filter, void* ws,
+              void* stream);
+
+/* Per-run max of |ux| + sqrt(gamma p / rho) -> max_dev[run] and, when dt_dev
+ * is not NULL, dt_dev[run] = cfl*min(dx,dy)/max (1e-6 if max <= 0).
+ * Replaces EulerEquations2D.max_wave_speed (phrike/equations.py:365-370,
+ * 137-147) and SpectralSolver2D.compute_dt (phrike/solver.py:613-620). */
+int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max_dev,
+                        double* dt_dev, double cfl, void* ws, void* stream);
+
+/* Per-run FP64 sums of rho, rho*ux, rho*uy, energy -> out4_dev[run*4 + i].
+ * Replaces the record() sums of SpectralSolver2D.run
+ * (phrike/solver.py:734-757). */
+int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out4_dev, void* ws,
+                        void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHRIKE_HIP_H */

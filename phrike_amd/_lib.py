@@ -1,0 +1,92 @@
+"""ctypes binding of libphrike_hip.so (include/phrike_hip.h).
+
+The library is built in-tree by ``python -m phrike_amd.build``.  There is no
+fallback: if the library is missing or cannot be loaded, every entry point
+raises, so a HIP-backend call never silently runs anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_size_t, c_void_p
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libphrike_hip.so")
+
+PH_F64 = 0
+PH_F32 = 1
+
+# symbol -> (restype, argtypes); mirrors include/phrike_hip.h
+_SIGNATURES = {
+    "ph_abi_version": (c_int, []),
+    "ph_last_error": (ctypes.c_char_p, []),
+    "ph_plan3d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_double, c_double,
+                                 c_int, c_int, c_int, c_int, c_double, c_double]),
+    "ph_plan3d_destroy": (c_int, [c_void_p]),
+    "ph_plan3d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_rhs3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_deriv3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_step3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "ph_max_wave_speed3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "ph_conserved_sums3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_plan2d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
+                                 c_int, c_int, c_double, c_double, c_int]),
+    "ph_plan2d_destroy": (c_int, [c_void_p]),
+    "ph_plan2d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_rhs2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_deriv2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_step2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
+                          c_void_p, c_void_p]),
+    "ph_max_wave_speed2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                    c_void_p]),
+    "ph_conserved_sums2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+class HipBackendError(RuntimeError):
+    """A failed C-ABI call (the library's message is attached)."""
+
+
+def load(path: str = LIB_PATH):
+    """Load the library once; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise HipBackendError(
+            f"{path} is missing: build it with `python -m phrike_amd.build` "
+            "(the hip backend has no fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def symbols():
+    return list(_SIGNATURES)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = load().ph_last_error()
+        msg = msg.decode() if msg else ""
+        raise HipBackendError(f"{what} failed ({rc}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def ptr(t) -> c_void_p:
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return c_void_p(0)
+    return c_void_p(t.data_ptr())
+
+
+def stream_handle(torch, device=None) -> c_void_p:
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)

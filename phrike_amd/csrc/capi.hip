@@ -1,0 +1,252 @@
+// C ABI (include/phrike_hip.h): argument checks, error strings, dispatch on
+// the plan's element type.
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "../../include/phrike_hip.h"
+#include "pipeline.hpp"
+
+namespace ph {
+extern template struct Plan3D<double>;
+extern template struct Plan3D<float>;
+extern template struct Plan2D<double>;
+extern template struct Plan2D<float>;
+}  // namespace ph
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return PH_OK;
+  } catch (const ph::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PH_ERR_INTERNAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return PH_ERR_INTERNAL;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw ph::Error(PH_ERR_ARG, msg);
+}
+
+ph::PlanBase* base3(ph_plan3d_t p) {
+  need(p != nullptr, "null plan");
+  return reinterpret_cast<ph::PlanBase*>(p);
+}
+ph::PlanBase* base2(ph_plan2d_t p) {
+  need(p != nullptr, "null plan");
+  return reinterpret_cast<ph::PlanBase*>(p);
+}
+
+template <typename F>
+void with3(ph_plan3d_t p, F&& f) {
+  ph::PlanBase* b = base3(p);
+  if (b->dtype == PH_F64) f(*static_cast<ph::Plan3D<double>*>(b));
+  else f(*static_cast<ph::Plan3D<float>*>(b));
+}
+template <typename F>
+void with2(ph_plan2d_t p, F&& f) {
+  ph::PlanBase* b = base2(p);
+  if (b->dtype == PH_F64) f(*static_cast<ph::Plan2D<double>*>(b));
+  else f(*static_cast<ph::Plan2D<float>*>(b));
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+}  // namespace
+
+extern "C" {
+
+int ph_abi_version(void) { return 1; }
+
+const char* ph_last_error(void) { return g_err.c_str(); }
+
+int ph_plan3d_create(ph_plan3d_t* plan, int nx, int ny, int nz, double lx, double ly, double lz,
+                     int dtype, int dealias, int filter_enabled, int filter_p, double filter_alpha,
+                     double gamma) {
+  return guard([&] {
+    need(plan != nullptr, "null plan pointer");
+    need(dtype == PH_F64 || dtype == PH_F32, "dtype must be PH_F64 or PH_F32");
+    need(lx > 0 && ly > 0 && lz > 0, "domain lengths must be positive");
+    need(gamma > 1.0, "gamma must exceed 1");
+    ph::PlanBase* p;
+    if (dtype == PH_F64)
+      p = new ph::Plan3D<double>(nx, ny, nz, lx, ly, lz, dealias != 0, filter_enabled != 0, filter_p,
+                                 filter_alpha, gamma);
+    else
+      p = new ph::Plan3D<float>(nx, ny, nz, lx, ly, lz, dealias != 0, filter_enabled != 0, filter_p,
+                                filter_alpha, gamma);
+    *plan = reinterpret_cast<ph_plan3d_t>(p);
+  });
+}
+
+int ph_plan3d_destroy(ph_plan3d_t plan) {
+  return guard([&] { delete base3(plan); });
+}
+
+int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes) {
+  return guard([&] {
+    need(bytes != nullptr, "null output");
+    with3(plan, [&](auto& p) { *bytes = p.wsBytes; });
+  });
+}
+
+int ph_rhs3d(ph_plan3d_t plan, const void* U, void* rhs, void* ws, void* stream) {
+  return guard([&] {
+    need(U && rhs && ws, "null buffer");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.rhs(static_cast<const R*>(U), static_cast<R*>(rhs), ws, S(stream));
+    });
+  });
+}
+
+int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out, void* ws,
+               void* stream) {
+  return guard([&] {
+    need(f && out && ws, "null buffer");
+    need(axis >= -1 && axis <= 2, "axis must be -1, 0, 1 or 2");
+    need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.deriv(axis, nfields, static_cast<const R*>(f), static_cast<R*>(out), ws, S(stream));
+    });
+  });
+}
+
+int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, void* ws,
+              void* stream) {
+  return guard([&] {
+    need(U && out && ws, "null buffer");
+    need(U != out, "U and out must not overlap");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, ws, S(stream));
+    });
+  });
+}
+
+int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double* dt_dev,
+                        double cfl, void* ws, void* stream) {
+  return guard([&] {
+    need(U && ws && (max_dev || dt_dev), "null buffer");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.max_speed(static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, S(stream));
+    });
+  });
+}
+
+int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void* ws, void* stream) {
+  return guard([&] {
+    need(U && out5_dev && ws, "null buffer");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.sums(static_cast<const R*>(U), out5_dev, ws, S(stream));
+    });
+  });
+}
+
+int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, int dtype, int dealias,
+                     int filter_enabled, int filter_p, double filter_alpha, double gamma, int batch) {
+  return guard([&] {
+    need(plan != nullptr, "null plan pointer");
+    need(dtype == PH_F64 || dtype == PH_F32, "dtype must be PH_F64 or PH_F32");
+    need(lx > 0 && ly > 0, "domain lengths must be positive");
+    need(gamma > 1.0, "gamma must exceed 1");
+    ph::PlanBase* p;
+    if (dtype == PH_F64)
+      p = new ph::Plan2D<double>(nx, ny, lx, ly, dealias != 0, filter_enabled != 0, filter_p,
+                                 filter_alpha, gamma, batch);
+    else
+      p = new ph::Plan2D<float>(nx, ny, lx, ly, dealias != 0, filter_enabled != 0, filter_p,
+                                filter_alpha, gamma, batch);
+    *plan = reinterpret_cast<ph_plan2d_t>(p);
+  });
+}
+
+int ph_plan2d_destroy(ph_plan2d_t plan) {
+  return guard([&] { delete base2(plan); });
+}
+
+int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes) {
+  return guard([&] {
+    need(bytes != nullptr, "null output");
+    with2(plan, [&](auto& p) { *bytes = p.wsBytes; });
+  });
+}
+
+#define PH_R2 \
+  using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan2D<double>>, double, float>
+
+int ph_rhs2d(ph_plan2d_t plan, int nbatch, const void* U, void* rhs, void* ws, void* stream) {
+  return guard([&] {
+    need(U && rhs && ws, "null buffer");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.rhs(nbatch, static_cast<const R*>(U), static_cast<R*>(rhs), ws, S(stream));
+    });
+  });
+}
+
+int ph_deriv2d(ph_plan2d_t plan, int nbatch, int axis, const void* f, void* out, void* ws,
+               void* stream) {
+  return guard([&] {
+    need(f && out && ws, "null buffer");
+    need(axis >= -1 && axis <= 1, "axis must be -1, 0 or 1");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.deriv(axis, nbatch, static_cast<const R*>(f), static_cast<R*>(out), nullptr, ws, S(stream));
+    });
+  });
+}
+
+int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt, const double* dt_dev,
+              const int* active_dev, int apply_filter, void* ws, void* stream) {
+  return guard([&] {
+    need(U && out && ws, "null buffer");
+    need(U != out, "U and out must not overlap");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev,
+             apply_filter != 0, ws, S(stream));
+    });
+  });
+}
+
+int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max_dev, double* dt_dev,
+                        double cfl, void* ws, void* stream) {
+  return guard([&] {
+    need(U && ws && (max_dev || dt_dev), "null buffer");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.max_speed(nbatch, static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, S(stream));
+    });
+  });
+}
+
+int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out4_dev, void* ws,
+                        void* stream) {
+  return guard([&] {
+    need(U && out4_dev && ws, "null buffer");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.sums(nbatch, static_cast<const R*>(U), out4_dev, ws, S(stream));
+    });
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+// Device FFT core for gfx950: radix-2/4/8 Stockham passes over a batch of
+// lines held in LDS.
+//
+// Layout in LDS: a batch of L lines of length N, element (line l, position n)
+// at buf[n * L + l]; consecutive threads take consecutive lines, so every
+// pass reads and writes LDS at unit stride across a wavefront.
+//
+// Pass p (radix R, stride Ns = product of earlier radices) takes butterfly j
+// (0 <= j < N/R) from positions j + r*N/R, multiplies input r by
+// exp(DIR*2*pi*i * r*(j % Ns)/(Ns*R)), runs a length-R DFT and stores output r
+// at (j/Ns)*Ns*R + (j % Ns) + r*Ns (Stockham autosort: output in natural
+// order, no bit reversal).  Every thread loads all of its butterflies'
+// inputs before a barrier and then stores, so the passes run in place.
+// The last pass can leave its outputs in registers: output r of butterfly j
+// is then position j + r*N/R.
+//
+// Twiddles come from a table tw[m] = exp(-2*pi*i*m/N) computed on the host in
+// FP64 (rounded to T for the FP32 path); the inverse uses conj(tw).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ph {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return mk(a.x * s, a.y * s); }
+// multiply by i*k (k real)
+template <typename T>
+__device__ __forceinline__ cx<T> mul_ik(cx<T> a, T k) { return mk(-a.y * k, a.x * k); }
+// multiply by DIR*i
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> rot(cx<T> a) {
+  return DIR < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cx<T>* v) {
+  cx<T> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cx<T>* v) {
+  cx<T> s02 = v[0] + v[2], d02 = v[0] - v[2];
+  cx<T> s13 = v[1] + v[3], d13 = rot<DIR>(v[1] - v[3]);
+  v[0] = s02 + s13;
+  v[2] = s02 - s13;
+  v[1] = d02 + d13;
+  v[3] = d02 - d13;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+  cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  const T c = T(0.70710678118654752440084436210485);
+  // o1 *= exp(DIR*i*pi/4), o2 *= DIR*i, o3 *= exp(DIR*3i*pi/4)
+  cx<T> o1 = mk(c * (o[1].x - T(DIR) * o[1].y), c * (o[1].y + T(DIR) * o[1].x));
+  cx<T> o2 = rot<DIR>(o[2]);
+  cx<T> o3 = mk(-c * (o[3].x + T(DIR) * o[3].y), c * (T(DIR) * o[3].x - o[3].y));
+  v[0] = e[0] + o[0];
+  v[4] = e[0] - o[0];
+  v[1] = e[1] + o1;
+  v[5] = e[1] - o1;
+  v[2] = e[2] + o2;
+  v[6] = e[2] - o2;
+  v[3] = e[3] + o3;
+  v[7] = e[3] - o3;
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dftR(cx<T>* v) {
+  if constexpr (R == 2) dft2<DIR>(v);
+  else if constexpr (R == 4) dft4<DIR>(v);
+  else dft8<DIR>(v);
+}
+
+constexpr int ilog2c(int n) { int k = 0; while ((1 << k) < n) ++k; return k; }
+
+// Radix plan: radix-8 passes, preceded by one radix-4 pass (log2 N = 3q+2) or
+// two radix-4 passes (log2 N = 3q+1, q >= 1); N = 2 is a single radix-2 pass.
+template <int N>
+struct Plan {
+  static_assert(N >= 2 && (N & (N - 1)) == 0, "power-of-two lengths only");
+  static constexpr int K = ilog2c(N);
+  static constexpr int n8 = K / 3;
+  static constexpr int rem = K % 3;
+  static constexpr int npass = rem == 0 ? n8 : (rem == 1 ? (n8 >= 1 ? n8 + 1 : 1) : n8 + 1);
+  static constexpr int radix(int p) {
+    return rem == 0 ? 8 : (rem == 1 ? (n8 >= 1 ? (p < 2 ? 4 : 8) : 2) : (p == 0 ? 4 : 8));
+  }
+};
+
+template <int N, int P>
+struct PassNs {  // product of the radices of passes 0..P-1
+  static constexpr int value = PassNs<N, P - 1>::value * Plan<N>::radix(P - 1);
+};
+template <int N>
+struct PassNs<N, 0> { static constexpr int value = 1; };
+
+template <int N, int L, int NT>
+struct LastRegs {
+  static constexpr int R = Plan<N>::radix(Plan<N>::npass - 1);
+  static constexpr int NB = L * (N / R);
+  static constexpr int PER = (NB + NT - 1) / NT;
+};
+
+// load the R inputs of butterfly b of pass (R, Ns) and apply the twiddles
+template <typename T, int N, int L, int DIR, int R, int NS>
+__device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int b, cx<T>* v) {
+  const int l = b % L, j = b / L;
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = buf[(j + r * (N / R)) * L + l];
+  if constexpr (NS > 1) {
+    const int k = j % NS;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      cx<T> w = tw[r * k * (N / (NS * R))];
+      if constexpr (DIR > 0) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  dftR<R, DIR>(v);
+}
+
+// one Stockham pass, LDS -> LDS (in place; ends with a barrier)
+template <typename T, int N, int L, int DIR, int NT, int P>
+__device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
+  constexpr int R = Plan<N>::radix(P);
+  constexpr int NS = PassNs<N, P>::value;
+  constexpr int NB = L * (N / R);
+  constexpr int PER = (NB + NT - 1) / NT;
+  cx<T> v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) {
+      const int l = b % L, j = b / L;
+      const int k = j % NS;
+      const int d = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(d + r * NS) * L + l] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int L, int DIR, int NT, int P, int PEND>
+__device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
+  if constexpr (P < PEND) {
+    pass_lds<T, N, L, DIR, NT, P>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND>(buf, tw, tid);
+  }
+}
+
+// Full transform of the L lines in buf, result in buf (natural order).
+// Caller must have synchronised after filling buf; ends with a barrier.
+template <typename T, int N, int L, int DIR, int NT>
+__device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
+  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass>(buf, tw, tid);
+}
+
+// Full transform with the last pass left in registers v[q][r]; output r of
+// butterfly b = tid + q*NT is line b % L, position b / L + r*N/R (last_pos).
+// A barrier separates the last pass's loads from the return, so the caller
+// may overwrite buf at once.
+template <typename T, int N, int L, int DIR, int NT>
+__device__ __forceinline__ void fft_regs(cx<T>* buf, const cx<T>* tw, int tid,
+                                         cx<T> (&v)[LastRegs<N, L, NT>::PER][LastRegs<N, L, NT>::R]) {
+  constexpr int LP = Plan<N>::npass - 1;
+  constexpr int R = Plan<N>::radix(LP);
+  constexpr int NS = PassNs<N, LP>::value;
+  constexpr int NB = L * (N / R);
+  constexpr int PER = LastRegs<N, L, NT>::PER;
+  passes_lds_range<T, N, L, DIR, NT, 0, LP>(buf, tw, tid);
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
+  }
+  __syncthreads();
+}
+
+// Is register slot q valid for this thread, and which (line, position) is
+// output r of it?
+template <int N, int L, int NT>
+__device__ __forceinline__ bool last_valid(int tid, int q) {
+  constexpr int NB = LastRegs<N, L, NT>::NB;
+  return (NB % NT == 0) || (tid + q * NT < NB);
+}
+template <int N, int L, int NT>
+__device__ __forceinline__ void last_pos(int tid, int q, int r, int& l, int& n) {
+  constexpr int R = LastRegs<N, L, NT>::R;
+  const int b = tid + q * NT;
+  l = b % L;
+  n = b / L + r * (N / R);
+}
+
+}  // namespace ph

@@ -1,0 +1,638 @@
+// Pencil kernels of the pseudo-spectral Euler RHS and RK4 step (gfx950).
+//
+// One RHS evaluation in 3D is five launches (the 2D form skips the z pass):
+//
+//   k_xfwd   flux (5 conserved -> 15 fluxes, in registers) + real-to-complex
+//            FFT along x of two y-rows at once (z = row0 + i*row1), keeps
+//            |mx| <= Nx/3, scales by sigma_x/(Nx*Ny*Nz) and multiplies the
+//            x-fluxes by i*kx                         -> A[15][z][y][mx]
+//   k_col    y pass: forward FFT along y of column tiles, keeps |my| <= Ny/3,
+//            forms G = sigma_y (Fx^ + i ky Fy^) and sigma_y Fz^
+//                                                     -> B[10][z][my][mx]
+//   k_col    z pass: forward FFT along z of G and Fz^, D = sigma_z (G + i kz
+//            Fz^) on |mz| <= Nz/3, zero elsewhere, inverse FFT along z
+//                                                     -> C[5][z][my][mx] (in B)
+//   k_col    y inverse: zero-filled inverse FFT along y -> D[5][z][y][mx] (in A)
+//   k_xinv   complex-to-real inverse along x of two rows at once, then the RK4
+//            stage combine (or plain output) in registers.
+//
+// Because differentiation, the 2/3 mask and the filter are all diagonal in
+// Fourier space, dx Fx + dy Fy + dz Fz is formed once in spectral space and
+// only 5 fields are transformed back: the same operator as three separate
+// masked derivatives, with 2/3 of the modes never written to HBM.
+//
+// All spectral buffers hold complex cx<T>; row pitch mxp >= mx keeps rows
+// aligned.  Every kernel uses 256 threads (four wave64s).
+#pragma once
+#include "fft_core.hpp"
+
+namespace ph {
+
+constexpr int NT = 256;  // reductions and small helpers
+
+constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
+// of the block's 2*NX points, which bounds the per-thread flux and FFT state.
+template <typename T, int NX>
+struct XThreads {
+  static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, 512) : clampi(NX / 2, 64, 256);
+};
+
+// ----------------------------------------------------------------------------
+// retained-mode bookkeeping along one axis: positions n <= kp and
+// n >= N - kn are kept; compact index j = n (n <= kp) or n - (N - nr).
+// ----------------------------------------------------------------------------
+struct Keep {
+  int kp, kn, nr;
+  __device__ __forceinline__ bool kept(int n, int N) const { return n <= kp || n >= N - kn; }
+  __device__ __forceinline__ int compact(int n, int N) const { return n <= kp ? n : n - (N - nr); }
+  __device__ __forceinline__ int natural(int j, int N) const { return j <= kp ? j : j + (N - nr); }
+};
+
+// ----------------------------------------------------------------------------
+// x forward
+// ----------------------------------------------------------------------------
+enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2 };
+
+template <typename T>
+struct XFwdArgs {
+  const T* in;
+  long long in_fs, in_ss;  // field / slab strides (elements); rows of nx
+  cx<T>* out;
+  long long out_fs, out_ss;  // rows of mxp complex
+  int nf;                    // XF_COPY: number of fields (<= 5)
+  int mx, mxp;
+  unsigned ikmask;  // bit f: multiply field f by i*kx
+  const cx<T>* tw;
+  const T* sig;  // sigma_x(m) * scale, m < mx
+  const T* kk;   // kx(m), zero at the Nyquist mode
+  T gamma;
+};
+
+template <int MODE> struct XFwdShape;
+template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
+template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
+template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
+
+template <typename T, int NX, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a) {
+  constexpr int NT = XThreads<T, NX>::value;
+  using S = XFwdShape<MODE>;
+  constexpr int LX = S::LX, NC = S::NC;
+  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  __shared__ cx<T> tw[NX];
+  __shared__ cx<T> buf[LX * NX];
+  T* bufr = reinterpret_cast<T*>(buf);
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  const int y0 = 2 * blockIdx.x;
+  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+
+  const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
+  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
+  T q[PPT][NC];
+#pragma unroll
+  for (int pi = 0; pi < PPT; ++pi) {
+    const int p = tid + pi * NT;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[pi][c] = T(0);
+    if (p < 2 * NX) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < ncomp) q[pi][c] = src[(long long)c * a.in_fs + p];
+    }
+  }
+  // primitive recovery (phrike/equations.py:104-117, 150-164)
+  T pr[PPT][6];
+  if constexpr (MODE != XF_COPY) {
+#pragma unroll
+    for (int pi = 0; pi < PPT; ++pi) {
+      const T rho = q[pi][0];
+      if constexpr (MODE == XF_FLUX3) {
+        const T ux = q[pi][1] / rho, uy = q[pi][2] / rho, uz = q[pi][3] / rho;
+        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+        const T p = (a.gamma - T(1)) * (q[pi][4] - kin);
+        pr[pi][0] = rho; pr[pi][1] = ux; pr[pi][2] = uy; pr[pi][3] = uz; pr[pi][4] = p;
+        pr[pi][5] = q[pi][4] + p;  // E + p
+      } else {
+        const T ux = q[pi][1] / rho, uy = q[pi][2] / rho;
+        const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+        const T p = (a.gamma - T(1)) * (q[pi][3] - kin);
+        pr[pi][0] = rho; pr[pi][1] = ux; pr[pi][2] = uy; pr[pi][3] = T(0); pr[pi][4] = p;
+        pr[pi][5] = q[pi][3] + p;
+      }
+    }
+  }
+
+  cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y0 * a.mxp;
+#pragma unroll 1
+  for (int g = 0; g < S::NG; ++g) {
+    // fill LX complex lines: line l holds field g*LX+l of row y0 (re) and y0+1 (im)
+#pragma unroll
+    for (int pi = 0; pi < PPT; ++pi) {
+      const int p = tid + pi * NT;
+      if (p < 2 * NX) {
+        const int line = p / NX, x = p - line * NX;
+#pragma unroll
+        for (int l = 0; l < LX; ++l) {
+          T v;
+          if constexpr (MODE == XF_COPY) {
+            v = q[pi][l];
+          } else {
+            // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
+            const T rho = pr[pi][0], p_ = pr[pi][4];
+            const T ud = pr[pi][1 + g];
+            const T md = rho * ud;
+            if (l == 0) v = md;
+            else if (l == NC - 1) v = pr[pi][5] * ud;
+            else {
+              v = md * pr[pi][l];
+              if (l == g + 1) v = v + p_;
+            }
+          }
+          bufr[2 * (x * LX + l) + line] = v;
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<T, NX, LX, -1, NT>(buf, tw, tid);
+    // split the two real spectra, keep mx modes, scale, write two rows
+    for (int i = tid; i < LX * a.mxp; i += NT) {
+      const int l = i / a.mxp, k = i - l * a.mxp;
+      const int f = g * LX + l;
+      if (MODE == XF_COPY && l >= ncomp) continue;
+      cx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
+      if (k < a.mx) {
+        const cx<T> zk = buf[k * LX + l];
+        const cx<T> zn = buf[((NX - k) & (NX - 1)) * LX + l];
+        A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
+        B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
+        const T s = a.sig[k];
+        A = cscale(A, s);
+        B = cscale(B, s);
+        if ((a.ikmask >> f) & 1u) {
+          const T kx = a.kk[k];
+          A = mul_ik(A, kx);
+          B = mul_ik(B, kx);
+        }
+      }
+      cx<T>* o = dst + (long long)f * a.out_fs + k;
+      o[0] = A;
+      o[a.mxp] = B;
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------------------
+// column passes (y and z): tiles of TM consecutive mx columns, lines along the
+// transform axis with element stride ls.
+// ----------------------------------------------------------------------------
+enum ColMode {
+  CM_FWD_DIV3 = 0,    // 3D y forward: A(15) -> B(10), compact out
+  CM_FWD_COPY = 1,    // forward, sigma (and i*k), compact out, nc fields
+  CM_FWDINV_DIV = 2,  // forward both halves, D = sigma (X + i k Z), inverse, full out
+  CM_FWDINV_COPY = 3, // forward, sigma (and i*k) on kept modes, inverse, full out
+  CM_INV = 4          // compact in, zero-filled inverse, full out
+};
+
+template <typename T>
+struct ColArgs {
+  const cx<T>* in;
+  long long in_fs, in_bs, in_ls;
+  cx<T>* out;
+  long long out_fs, out_bs, out_ls;
+  int ncol;   // valid columns
+  int nc;     // output fields
+  Keep keep;  // retained modes along this axis
+  int ikflag; // COPY modes: multiply by i*k
+  const cx<T>* tw;
+  const T* sig;  // by compact index
+  const T* kk;   // by compact index
+};
+
+template <typename T, int N>
+struct ColTile {
+  static constexpr int ELEMS = (sizeof(T) == 8) ? 4096 : 8192;
+  static constexpr int TM = clampi(ELEMS / N, 1, 64);
+  // 8 (FP64) or 16 (FP32) complex values of each register array per thread
+  static constexpr int NT = clampi((N * TM) / (sizeof(T) == 8 ? 8 : 16), 64, 512);
+};
+
+template <typename T, int N, int TM, int NT>
+__device__ __forceinline__ void col_load_full(cx<T>* buf, const cx<T>* src, long long ls, int c0,
+                                              int ncol, int tid) {
+  for (int i = tid; i < N * TM; i += NT) {
+    const int n = i / TM, l = i - n * TM;
+    const int col = c0 + l;
+    buf[i] = (col < ncol) ? src[(long long)n * ls + col] : mk(T(0), T(0));
+  }
+}
+
+template <typename T, int N, int TM, int NT>
+__device__ __forceinline__ void col_load_compact(cx<T>* buf, const cx<T>* src, long long ls, int c0,
+                                                 int ncol, const Keep& kp, int tid) {
+  for (int i = tid; i < N * TM; i += NT) {
+    const int n = i / TM, l = i - n * TM;
+    const int col = c0 + l;
+    cx<T> v = mk(T(0), T(0));
+    if (col < ncol && kp.kept(n, N)) v = src[(long long)kp.compact(n, N) * ls + col];
+    buf[i] = v;
+  }
+}
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__((ColTile<T, N>::NT)) void k_col(ColArgs<T> a) {
+  constexpr int TM = ColTile<T, N>::TM;
+  constexpr int NT = ColTile<T, N>::NT;
+  using LR = LastRegs<N, TM, NT>;
+  constexpr int PER = LR::PER, R = LR::R;
+  __shared__ cx<T> tw[N];
+  __shared__ cx<T> buf[N * TM];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * TM;
+  for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
+  const cx<T>* in = a.in + (long long)blockIdx.y * a.in_bs;
+  cx<T>* out = a.out + (long long)blockIdx.y * a.out_bs;
+  const Keep kp = a.keep;
+  cx<T> X[PER][R], Z[PER][R];
+
+#pragma unroll 1
+  for (int c = 0; c < a.nc; ++c) {
+    if constexpr (MODE == CM_INV) {
+      col_load_compact<T, N, TM, NT>(buf, in + (long long)c * a.in_fs, a.in_ls, c0, a.ncol, kp, tid);
+      __syncthreads();
+      fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
+    } else {
+      col_load_full<T, N, TM, NT>(buf, in + (long long)c * a.in_fs, a.in_ls, c0, a.ncol, tid);
+      __syncthreads();
+      fft_regs<T, N, TM, -1, NT>(buf, tw, tid, X);
+      if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) {
+        const int second = (MODE == CM_FWD_DIV3) ? 5 + c : a.nc + c;
+        col_load_full<T, N, TM, NT>(buf, in + (long long)second * a.in_fs, a.in_ls, c0, a.ncol, tid);
+        __syncthreads();
+        fft_regs<T, N, TM, -1, NT>(buf, tw, tid, Z);
+      }
+    }
+    // combine on the forward outputs (all modes but CM_INV)
+    if constexpr (MODE != CM_INV) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int l, n;
+          last_pos<N, TM, NT>(tid, q, r, l, n);
+          const bool kept = kp.kept(n, N);
+          cx<T> v = mk(T(0), T(0));
+          if (kept) {
+            const int j = kp.compact(n, N);
+            const T s = a.sig[j];
+            if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) {
+              v = cscale(X[q][r] + mul_ik(Z[q][r], a.kk[j]), s);
+            } else {
+              v = cscale(X[q][r], s);
+              if (a.ikflag) v = mul_ik(v, a.kk[j]);
+            }
+          }
+          X[q][r] = v;
+        }
+      }
+    }
+    if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWD_COPY) {
+      cx<T>* o = out + (long long)c * a.out_fs;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int l, n;
+          last_pos<N, TM, NT>(tid, q, r, l, n);
+          const int col = c0 + l;
+          if (col < a.ncol && kp.kept(n, N)) o[(long long)kp.compact(n, N) * a.out_ls + col] = X[q][r];
+        }
+      }
+      if constexpr (MODE == CM_FWD_DIV3) {
+        // sigma_y * Fz^ (second output group)
+        col_load_full<T, N, TM, NT>(buf, in + (long long)(10 + c) * a.in_fs, a.in_ls, c0, a.ncol, tid);
+        __syncthreads();
+        fft_regs<T, N, TM, -1, NT>(buf, tw, tid, Z);
+        cx<T>* o2 = out + (long long)(5 + c) * a.out_fs;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            int l, n;
+            last_pos<N, TM, NT>(tid, q, r, l, n);
+            const int col = c0 + l;
+            if (col < a.ncol && kp.kept(n, N)) {
+              const int j = kp.compact(n, N);
+              o2[(long long)j * a.out_ls + col] = cscale(Z[q][r], a.sig[j]);
+            }
+          }
+        }
+      }
+    } else {
+      if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_COPY) {
+        // spectral result back to LDS in natural order, then inverse
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            int l, n;
+            last_pos<N, TM, NT>(tid, q, r, l, n);
+            buf[n * TM + l] = X[q][r];
+          }
+        }
+        __syncthreads();
+        fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
+      }
+      cx<T>* o = out + (long long)c * a.out_fs;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int l, n;
+          last_pos<N, TM, NT>(tid, q, r, l, n);
+          const int col = c0 + l;
+          if (col < a.ncol) o[(long long)n * a.out_ls + col] = X[q][r];
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// x inverse + stage combine
+// ----------------------------------------------------------------------------
+enum XInvMode {
+  XI_OUT = 0,      // out = value
+  XI_RHS = 1,      // out = -value
+  XI_STAGE3 = 2,   // 3D RK4 stage combine
+  XI_STAGE2 = 3    // 2D RK4 stage combine with positivity clamp
+};
+
+template <typename T>
+struct XInvArgs {
+  const cx<T>* in;
+  long long in_fs, in_ss;  // rows of mxp
+  int mx, mxp;
+  int nf;                  // XI_OUT / XI_RHS: fields (<= NC)
+  T* out;                  // output (and the RK accumulator in stage modes)
+  const T* un;             // state at the start of the step
+  T* us;                   // next stage state
+  long long fs, ss;        // physical field / slab strides (elements); rows of nx
+  int stage;               // 1..4
+  double dt;               // used when dt_dev is null
+  const double* dt_dev;    // per-slab dt (batched 2D) or null
+  const int* active;       // per-slab flag (batched 2D) or null
+  T gamma, rho_min, p_min;
+  const cx<T>* tw;
+};
+
+template <typename T>
+__device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
+  // phrike/solver.py:467-488: (rho,ux,uy,p) -> clamp rho, p -> conservative
+  const T rho = u[0];
+  const T ux = u[1] / rho, uy = u[2] / rho;
+  const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+  const T p = (gamma - T(1)) * (u[3] - kin);
+  const T rc = rho < rho_min ? rho_min : rho;  // NaN propagates like np.maximum
+  const T pc = p < p_min ? p_min : p;
+  u[0] = rc;
+  u[1] = rc * ux;
+  u[2] = rc * uy;
+  u[3] = pc / (gamma - T(1)) + (T(0.5) * rc) * (ux * ux + uy * uy);
+}
+
+template <typename T, int NX, int NC, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a) {
+  constexpr int NT = XThreads<T, NX>::value;
+  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  __shared__ cx<T> tw[NX];
+  __shared__ cx<T> buf[NC * NX];
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (a.active != nullptr && a.active[slab] == 0) return;
+  const int y0 = 2 * blockIdx.x;
+  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
+  const cx<T>* src = a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp;
+  // rebuild z = row0 + i*row1 from the kept half spectra of both rows
+  for (int i = tid; i < NC * NX; i += NT) {
+    const int l = i / NX, n = i - l * NX;
+    cx<T> z = mk(T(0), T(0));
+    if (l < nfl) {
+      const cx<T>* s = src + (long long)l * a.in_fs;
+      if (n < a.mx) {
+        const cx<T> A = s[n], B = s[a.mxp + n];
+        z = (n == 0 || 2 * n == NX) ? mk(A.x, B.x) : mk(A.x - B.y, A.y + B.x);
+      } else if (n > NX - a.mx) {
+        const int k = NX - n;
+        const cx<T> A = s[k], B = s[a.mxp + k];
+        z = mk(A.x + B.y, B.x - A.y);
+      }
+    }
+    buf[n * NC + l] = z;
+  }
+  __syncthreads();
+  fft_lds<T, NX, NC, +1, NT>(buf, tw, tid);
+  const T* bufr = reinterpret_cast<const T*>(buf);
+  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
+  double dtd = a.dt;
+  if (a.dt_dev != nullptr) dtd = a.dt_dev[slab];
+#pragma unroll
+  for (int pi = 0; pi < PPT; ++pi) {
+    const int p = tid + pi * NT;
+    if (p >= 2 * NX) continue;
+    const int line = p / NX, x = p - line * NX;
+    const long long o = base + p;  // rows y0, y0+1 are contiguous
+    T v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * NC + c) + line];
+    if constexpr (MODE == XI_OUT) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < nfl) a.out[c * a.fs + o] = v[c];
+    } else if constexpr (MODE == XI_RHS) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < nfl) a.out[c * a.fs + o] = -v[c];
+    } else {
+      // phrike/solver.py:1220-1230 (3D), 502-513 (2D): k_s = -value
+      const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0);
+      T u[NC];
+      if (a.stage < 4) {
+        const T coef = (a.stage == 3) ? full : half;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const T k = -v[c];
+          const long long idx = c * a.fs + o;
+          const T un = a.un[idx];
+          if (a.stage == 1) a.out[idx] = k;
+          else a.out[idx] = a.out[idx] + T(2) * k;
+          u[c] = un + coef * k;
+        }
+        if constexpr (MODE == XI_STAGE2) clamp2d(u, a.gamma, a.rho_min, a.p_min);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a.us[c * a.fs + o] = u[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const long long idx = c * a.fs + o;
+          u[c] = a.un[idx] + sixth * (a.out[idx] + (-v[c]));
+        }
+        if constexpr (MODE == XI_STAGE2) clamp2d(u, a.gamma, a.rho_min, a.p_min);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// reductions: per-block partials, then one block reduces them in a fixed
+// order (deterministic; no atomics, no memset).
+// ----------------------------------------------------------------------------
+template <typename T, int DIM>
+__device__ __forceinline__ double wave_speed(const T* u, long long fs, long long i, T gamma) {
+  // 3D: max(|ux|,|uy|,|uz|) + a (phrike/equations.py:199-211);
+  // 2D: |ux| + a (phrike/equations.py:137-147)
+  const T rho = u[i];
+  if constexpr (DIM == 3) {
+    const T ux = u[fs + i] / rho, uy = u[2 * fs + i] / rho, uz = u[3 * fs + i] / rho;
+    const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+    const T p = (gamma - T(1)) * (u[4 * fs + i] - kin);
+    const T a = sqrt(gamma * p / rho);
+    T m = fabs(ux) > fabs(uy) ? fabs(ux) : fabs(uy);
+    m = m > fabs(uz) ? m : fabs(uz);
+    return double(m + a);
+  } else {
+    const T ux = u[fs + i] / rho, uy = u[2 * fs + i] / rho;
+    const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+    const T p = (gamma - T(1)) * (u[3 * fs + i] - kin);
+    const T a = sqrt(gamma * p / rho);
+    return double(fabs(ux) + a);
+  }
+}
+
+__device__ __forceinline__ double dmax_nan(double a, double b) {
+  // NaN wins (a NaN speed must not be hidden)
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long fs, long long ss,
+                                                         long long n, T gamma, double* part) {
+  // grid: (blocks per slab, slabs); part[slab * gridDim.x + blockIdx.x]
+  __shared__ double red[NT];
+  const T* us = u + (long long)blockIdx.y * ss;
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
+    m = dmax_nan(m, wave_speed<T, DIM>(us, fs, i, gamma));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = dmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// one block per slab: reduce nb partials -> out[slab]; optionally dt = cfl*h/max
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_max_final(const double* part, int nb, double* out,
+                                                  double* dt_out, double cfl_h) {
+  __shared__ double red[NT];
+  const double* ps = part + (long long)blockIdx.x * nb;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) m = dmax_nan(m, ps[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = dmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (out) out[blockIdx.x] = red[0];
+    if (dt_out) dt_out[blockIdx.x] = red[0] > 0.0 ? cfl_h / red[0] : 1e-6;
+  }
+}
+
+// conserved sums: 3D mass, mx, my, mz, energy from primitives
+// (phrike/solver.py:1439-1468); 2D mass, mx, my, energy (solver.py:734-757)
+template <typename T, int DIM>
+__global__ __launch_bounds__(NT) void k_sums_partial(const T* u, long long fs, long long ss,
+                                                     long long n, T gamma, double* part) {
+  constexpr int NS = DIM == 3 ? 5 : 4;
+  __shared__ double red[NS][NT];
+  const T* us = u + (long long)blockIdx.y * ss;
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+    const T rho = us[i];
+    if constexpr (DIM == 3) {
+      const T ux = us[fs + i] / rho, uy = us[2 * fs + i] / rho, uz = us[3 * fs + i] / rho;
+      const T q2 = (ux * ux + uy * uy) + uz * uz;
+      const T p = (gamma - T(1)) * (us[4 * fs + i] - (T(0.5) * rho) * q2);
+      acc[0] += double(rho);
+      acc[1] += double(rho * ux);
+      acc[2] += double(rho * uy);
+      acc[3] += double(rho * uz);
+      acc[4] += double(p / (gamma - T(1)) + (T(0.5) * rho) * q2);
+    } else {
+      const T ux = us[fs + i] / rho, uy = us[2 * fs + i] / rho;
+      const T q2 = ux * ux + uy * uy;
+      const T p = (gamma - T(1)) * (us[3 * fs + i] - (T(0.5) * rho) * q2);
+      acc[0] += double(rho);
+      acc[1] += double(rho * ux);
+      acc[2] += double(rho * uy);
+      acc[3] += double(p / (gamma - T(1)) + (T(0.5) * rho) * q2);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) red[s][threadIdx.x] = acc[s];
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) red[s][threadIdx.x] += red[s][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NS)
+    part[((long long)blockIdx.y * gridDim.x + blockIdx.x) * NS + threadIdx.x] = red[threadIdx.x][0];
+}
+
+template <int NS>
+__global__ __launch_bounds__(NT) void k_sums_final(const double* part, int nb, double* out) {
+  __shared__ double red[NS][NT];
+  const double* ps = part + (long long)blockIdx.x * nb * NS;
+  double acc[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) acc[s] = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] += ps[(long long)i * NS + s];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) red[s][threadIdx.x] = acc[s];
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) red[s][threadIdx.x] += red[s][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < NS) out[(long long)blockIdx.x * NS + threadIdx.x] = red[threadIdx.x][0];
+}
+
+}  // namespace ph

@@ -1,0 +1,539 @@
+// Host-side plans and launch sequences for the 3D and batched-2D pipelines.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace ph {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(-5, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+// Retained modes of one axis and its FP64 tables (phrike/grid.py:87-141,
+// 1381-1392): k = 2*pi*(m*(1/(N*d))) with d = L/N in fftfreq order,
+// 2/3 rule |m| <= N//3, sigma = exp(-alpha*(|m|/max(N/2,1))^p).
+struct AxisTables {
+  int n = 0;
+  int kp = 0, kn = 0, nr = 0;  // kept positions n <= kp and n >= N-kn
+  std::vector<double> k, sig;  // by compact index (x: m = 0..kp)
+  std::vector<double> twr, twi;
+
+  AxisTables() = default;
+  AxisTables(int N, double L, bool dealias, bool filt, int p, double alpha, bool half) : n(N) {
+    if (dealias) {
+      kp = N / 3;
+      kn = N / 3;
+    } else {
+      kp = N / 2;
+      kn = N / 2 - 1;
+    }
+    if (half) kn = 0;
+    nr = kp + 1 + kn;
+    const double d = L / N;
+    const double inv = 1.0 / (N * d);
+    const double mmax = std::max(double(N / 2), 1.0);
+    for (int j = 0; j < nr; ++j) {
+      const int nat = j <= kp ? j : j + (N - nr);
+      const int m = nat < (N + 1) / 2 ? nat : nat - N;  // fftfreq ordering
+      double kv = 2.0 * M_PI * (double(m) * inv);
+      if (2 * nat == N) kv = 0.0;  // Nyquist: i*k gives no real part
+      k.push_back(kv);
+      double s = 1.0;
+      if (filt) s = std::exp(-alpha * std::pow(std::fabs(double(m)) / mmax, double(p)));
+      sig.push_back(s);
+    }
+    twr.resize(N);
+    twi.resize(N);
+    for (int m = 0; m < N; ++m) {
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / N;
+      twr[m] = (double)cosl(a);
+      twi[m] = (double)sinl(a);
+    }
+  }
+  Keep keep() const { return Keep{kp, kn, nr}; }
+};
+
+template <typename T>
+struct DevAxis {
+  cx<T>* tw = nullptr;
+  T* sig = nullptr;
+  T* k = nullptr;
+  void upload(const AxisTables& a, double scale) {
+    std::vector<cx<T>> tw_h(a.n);
+    for (int m = 0; m < a.n; ++m) {
+      tw_h[m].x = T(a.twr[m]);
+      tw_h[m].y = T(a.twi[m]);
+    }
+    std::vector<T> s_h(a.nr), k_h(a.nr);
+    for (int j = 0; j < a.nr; ++j) {
+      s_h[j] = T(a.sig[j] * scale);
+      k_h[j] = T(a.k[j]);
+    }
+    hip_check(hipMalloc(&tw, sizeof(cx<T>) * a.n), "hipMalloc(twiddles)");
+    hip_check(hipMalloc(&sig, sizeof(T) * a.nr), "hipMalloc(sigma)");
+    hip_check(hipMalloc(&k, sizeof(T) * a.nr), "hipMalloc(k)");
+    hip_check(hipMemcpy(tw, tw_h.data(), sizeof(cx<T>) * a.n, hipMemcpyHostToDevice), "copy twiddles");
+    hip_check(hipMemcpy(sig, s_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy sigma");
+    hip_check(hipMemcpy(k, k_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy k");
+  }
+  void release() {
+    if (tw) (void)hipFree(tw);
+    if (sig) (void)hipFree(sig);
+    if (k) (void)hipFree(k);
+    tw = nullptr;
+    sig = nullptr;
+    k = nullptr;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// launch helpers with compile-time lengths
+// ---------------------------------------------------------------------------
+#define PH_SWITCH_N(NV, BODY)                                     \
+  switch (NV) {                                                   \
+    case 8: { constexpr int N_ = 8; BODY; } break;                \
+    case 16: { constexpr int N_ = 16; BODY; } break;              \
+    case 32: { constexpr int N_ = 32; BODY; } break;              \
+    case 64: { constexpr int N_ = 64; BODY; } break;              \
+    case 128: { constexpr int N_ = 128; BODY; } break;            \
+    case 256: { constexpr int N_ = 256; BODY; } break;            \
+    case 512: { constexpr int N_ = 512; BODY; } break;            \
+    case 1024: { constexpr int N_ = 1024; BODY; } break;          \
+    default: throw Error(-2, "unsupported axis length " + std::to_string(NV)); \
+  }
+
+inline bool supported_len(int n) { return n >= 8 && n <= 1024 && is_pow2(n); }
+
+template <typename T, int MODE>
+void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t st) {
+  dim3 grid(nrows / 2, nslab);
+  PH_SWITCH_N(nx, (k_xfwd<T, N_, MODE><<<grid, XThreads<T, N_>::value, 0, st>>>(a)));
+}
+
+template <typename T, int MODE>
+void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
+  PH_SWITCH_N(n, ({
+    constexpr int TM = ColTile<T, N_>::TM;
+    dim3 grid((a.ncol + TM - 1) / TM, nblocky);
+    k_col<T, N_, MODE><<<grid, ColTile<T, N_>::NT, 0, st>>>(a);
+  }));
+}
+
+template <typename T, int NC, int MODE>
+void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t st) {
+  dim3 grid(nrows / 2, nslab);
+  PH_SWITCH_N(nx, (k_xinv<T, N_, NC, MODE><<<grid, XThreads<T, N_>::value, 0, st>>>(a)));
+}
+
+inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+// ---------------------------------------------------------------------------
+// 3D plan
+// ---------------------------------------------------------------------------
+struct PlanBase {
+  virtual ~PlanBase() = default;
+  int dtype = 0;  // 0 = float64, 1 = float32
+};
+
+template <typename T>
+struct Plan3D : PlanBase {
+  int nx, ny, nz;
+  double lx, ly, lz, gamma;
+  AxisTables ax, ay, az;
+  DevAxis<T> dx, dy, dz;
+  int mx, mxp;
+  long long ncell;
+  // workspace layout (bytes)
+  size_t offA, offB, offUs, offPart, wsBytes;
+  static constexpr int NPART = 1024;
+
+  Plan3D(int nx_, int ny_, int nz_, double lx_, double ly_, double lz_, bool dealias, bool filt,
+         int p, double alpha, double gamma_)
+      : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_) {
+    dtype = sizeof(T) == 8 ? 0 : 1;
+    if (!supported_len(nx) || !supported_len(ny) || !supported_len(nz))
+      throw Error(-2, "grid sizes must be powers of two in [8, 1024]");
+    ax = AxisTables(nx, lx, dealias, filt, p, alpha, true);
+    ay = AxisTables(ny, ly, dealias, filt, p, alpha, false);
+    az = AxisTables(nz, lz, dealias, filt, p, alpha, false);
+    mx = ax.nr;
+    const int align = sizeof(T) == 8 ? 4 : 8;  // 64-byte rows
+    mxp = (mx + align - 1) / align * align;
+    ncell = (long long)nx * ny * nz;
+    const double scale = 1.0 / (double(nx) * double(ny) * double(nz));
+    dx.upload(ax, scale);
+    dy.upload(ay, 1.0);
+    dz.upload(az, 1.0);
+    const size_t cs = sizeof(cx<T>);
+    const size_t szA = 15ull * nz * ny * mxp * cs;
+    const size_t szB = 10ull * nz * ay.nr * mxp * cs;
+    const size_t szU = 5ull * ncell * sizeof(T);
+    auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+    offA = 0;
+    offB = al(offA + szA);
+    offUs = al(offB + szB);
+    offPart = al(offUs + szU);
+    wsBytes = al(offPart + sizeof(double) * NPART * 5);
+  }
+  ~Plan3D() override {
+    dx.release();
+    dy.release();
+    dz.release();
+  }
+
+  cx<T>* A(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offA); }
+  cx<T>* B(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offB); }
+  T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
+  double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
+
+  long long a_fs() const { return (long long)nz * ny * mxp; }
+  long long b_fs() const { return (long long)nz * ay.nr * mxp; }
+
+  // forward x pass
+  void xfwd(int mode, const T* in, int nf, unsigned ikmask, void* ws, hipStream_t st) const {
+    XFwdArgs<T> a{};
+    a.in = in;
+    a.in_fs = ncell;
+    a.in_ss = (long long)ny * nx;
+    a.out = A(ws);
+    a.out_fs = a_fs();
+    a.out_ss = (long long)ny * mxp;
+    a.nf = nf;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.ikmask = ikmask;
+    a.tw = dx.tw;
+    a.sig = dx.sig;
+    a.kk = dx.k;
+    a.gamma = T(gamma);
+    if (mode == XF_FLUX3) launch_xfwd<T, XF_FLUX3>(nx, ny, nz, a, st);
+    else launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
+    check_launch("k_xfwd");
+  }
+  // y forward: A -> B
+  void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
+    ColArgs<T> a{};
+    a.in = A(ws);
+    a.in_fs = a_fs();
+    a.in_bs = (long long)ny * mxp;
+    a.in_ls = mxp;
+    a.out = B(ws);
+    a.out_fs = b_fs();
+    a.out_bs = (long long)ay.nr * mxp;
+    a.out_ls = mxp;
+    a.ncol = mx;
+    a.nc = nc;
+    a.keep = ay.keep();
+    a.ikflag = ikflag;
+    a.tw = dy.tw;
+    a.sig = dy.sig;
+    a.kk = dy.k;
+    if (div) launch_col<T, CM_FWD_DIV3>(ny, nz, a, st);
+    else launch_col<T, CM_FWD_COPY>(ny, nz, a, st);
+    check_launch("k_col(y fwd)");
+  }
+  // z forward + combine + inverse, in place in B
+  void zpass(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
+    ColArgs<T> a{};
+    a.in = B(ws);
+    a.in_fs = b_fs();
+    a.in_bs = mxp;
+    a.in_ls = (long long)ay.nr * mxp;
+    a.out = B(ws);
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = mx;
+    a.nc = nc;
+    a.keep = az.keep();
+    a.ikflag = ikflag;
+    a.tw = dz.tw;
+    a.sig = dz.sig;
+    a.kk = dz.k;
+    if (div) launch_col<T, CM_FWDINV_DIV>(nz, ay.nr, a, st);
+    else launch_col<T, CM_FWDINV_COPY>(nz, ay.nr, a, st);
+    check_launch("k_col(z)");
+  }
+  // y inverse: B (compact) -> A (full)
+  void yinv(int nc, void* ws, hipStream_t st) const {
+    ColArgs<T> a{};
+    a.in = B(ws);
+    a.in_fs = b_fs();
+    a.in_bs = (long long)ay.nr * mxp;
+    a.in_ls = mxp;
+    a.out = A(ws);
+    a.out_fs = a_fs();
+    a.out_bs = (long long)ny * mxp;
+    a.out_ls = mxp;
+    a.ncol = mx;
+    a.nc = nc;
+    a.keep = ay.keep();
+    a.ikflag = 0;
+    a.tw = dy.tw;
+    a.sig = dy.sig;
+    a.kk = dy.k;
+    launch_col<T, CM_INV>(ny, nz, a, st);
+    check_launch("k_col(y inv)");
+  }
+  XInvArgs<T> xinv_args(void* ws) const {
+    XInvArgs<T> a{};
+    a.in = A(ws);
+    a.in_fs = a_fs();
+    a.in_ss = (long long)ny * mxp;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.fs = ncell;
+    a.ss = (long long)ny * nx;
+    a.gamma = T(gamma);
+    a.tw = dx.tw;
+    return a;
+  }
+
+  // ---- public operations ----
+  void spectral(const T* in, int nf, int axis, void* ws, hipStream_t st) const {
+    // fields -> D (in A): transform, sigma*mask (and i*k along `axis` if >= 0)
+    xfwd(XF_COPY, in, nf, axis == 0 ? 0x1Fu : 0u, ws, st);
+    yfwd(false, nf, axis == 1, ws, st);
+    zpass(false, nf, axis == 2, ws, st);
+    yinv(nf, ws, st);
+  }
+  void deriv(int axis, int nf, const T* in, T* out, void* ws, hipStream_t st) const {
+    spectral(in, nf, axis, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = nf;
+    a.out = out;
+    launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+    check_launch("k_xinv");
+  }
+  void divergence(const T* U, void* ws, hipStream_t st) const {
+    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st);
+    yfwd(true, 5, 0, ws, st);
+    zpass(true, 5, 0, ws, st);
+    yinv(5, ws, st);
+  }
+  void rhs(const T* U, T* out, void* ws, hipStream_t st) const {
+    divergence(U, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = 5;
+    a.out = out;
+    launch_xinv<T, 5, XI_RHS>(nx, ny, nz, a, st);
+    check_launch("k_xinv");
+  }
+  // RK4 (phrike/solver.py:1220-1230) then the post-step filter (1233-1234,
+  // 1336).  `out` doubles as the stage accumulator.
+  void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st) const {
+    T* us = Us(ws);
+    for (int s = 1; s <= 4; ++s) {
+      divergence(s == 1 ? U : us, ws, st);
+      XInvArgs<T> a = xinv_args(ws);
+      a.nf = 5;
+      a.out = out;
+      a.un = U;
+      a.us = us;
+      a.stage = s;
+      a.dt = dt;
+      a.dt_dev = dt_dev;
+      launch_xinv<T, 5, XI_STAGE3>(nx, ny, nz, a, st);
+      check_launch("k_xinv(stage)");
+    }
+    deriv(-1, 5, out, out, ws, st);
+  }
+  void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
+    double* part = Part(ws);
+    const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    k_maxspeed_partial<T, 3><<<dim3(nb, 1), NT, 0, st>>>(U, ncell, 0, ncell, T(gamma), part);
+    check_launch("k_maxspeed_partial");
+    const double h = std::min(lx / nx, std::min(ly / ny, lz / nz));
+    k_max_final<3><<<1, NT, 0, st>>>(part, nb, out, dt_out, cfl * h);
+    check_launch("k_max_final");
+  }
+  void sums(const T* U, double* out, void* ws, hipStream_t st) const {
+    double* part = Part(ws);
+    const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    k_sums_partial<T, 3><<<dim3(nb, 1), NT, 0, st>>>(U, ncell, 0, ncell, T(gamma), part);
+    check_launch("k_sums_partial");
+    k_sums_final<5><<<1, NT, 0, st>>>(part, nb, out);
+    check_launch("k_sums_final");
+  }
+};
+
+// ---------------------------------------------------------------------------
+// batched 2D Fourier x Fourier plan: `batch` independent runs of one shape,
+// state (batch, 4, ny, nx)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Plan2D : PlanBase {
+  int nx, ny, batch;
+  double lx, ly, gamma;
+  AxisTables ax, ay;
+  DevAxis<T> dx, dy;
+  int mx, mxp;
+  long long ncell;
+  size_t offA, offUs, offPart, wsBytes;
+  static constexpr int NPART = 256;
+
+  Plan2D(int nx_, int ny_, double lx_, double ly_, bool dealias, bool filt, int p, double alpha,
+         double gamma_, int batch_)
+      : nx(nx_), ny(ny_), batch(batch_), lx(lx_), ly(ly_), gamma(gamma_) {
+    dtype = sizeof(T) == 8 ? 0 : 1;
+    if (!supported_len(nx) || !supported_len(ny))
+      throw Error(-2, "grid sizes must be powers of two in [8, 1024]");
+    if (batch < 1 || batch > 65535) throw Error(-2, "batch must be in [1, 65535]");
+    ax = AxisTables(nx, lx, dealias, filt, p, alpha, true);
+    ay = AxisTables(ny, ly, dealias, filt, p, alpha, false);
+    mx = ax.nr;
+    const int align = sizeof(T) == 8 ? 4 : 8;
+    mxp = (mx + align - 1) / align * align;
+    ncell = (long long)nx * ny;
+    dx.upload(ax, 1.0 / (double(nx) * double(ny)));
+    dy.upload(ay, 1.0);
+    const size_t cs = sizeof(cx<T>);
+    auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+    offA = 0;
+    offUs = al(8ull * batch * ny * mxp * cs);
+    offPart = al(offUs + 4ull * batch * ncell * sizeof(T));
+    wsBytes = al(offPart + sizeof(double) * (size_t)NPART * 5 * batch);
+  }
+  ~Plan2D() override {
+    dx.release();
+    dy.release();
+  }
+  cx<T>* A(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offA); }
+  T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
+  double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
+
+  void check_nb(int nb) const {
+    if (nb < 1 || nb > batch) throw Error(-2, "nbatch must be in [1, plan batch]");
+  }
+  void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st) const {
+    XFwdArgs<T> a{};
+    a.in = in;
+    a.in_fs = ncell;
+    a.in_ss = 4 * ncell;
+    a.out = A(ws);
+    a.out_fs = (long long)ny * mxp;
+    a.out_ss = 8ll * ny * mxp;
+    a.nf = 4;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.ikmask = ikmask;
+    a.tw = dx.tw;
+    a.sig = dx.sig;
+    a.kk = dx.k;
+    a.gamma = T(gamma);
+    if (mode == XF_FLUX2) launch_xfwd<T, XF_FLUX2>(nx, ny, nb, a, st);
+    else launch_xfwd<T, XF_COPY>(nx, ny, nb, a, st);
+    check_launch("k_xfwd(2d)");
+  }
+  void ypass(bool div, int ikflag, int nb, void* ws, hipStream_t st) const {
+    ColArgs<T> a{};
+    a.in = A(ws);
+    a.in_fs = (long long)ny * mxp;
+    a.in_bs = 8ll * ny * mxp;
+    a.in_ls = mxp;
+    a.out = A(ws);
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = mx;
+    a.nc = 4;
+    a.keep = ay.keep();
+    a.ikflag = ikflag;
+    a.tw = dy.tw;
+    a.sig = dy.sig;
+    a.kk = dy.k;
+    if (div) launch_col<T, CM_FWDINV_DIV>(ny, nb, a, st);
+    else launch_col<T, CM_FWDINV_COPY>(ny, nb, a, st);
+    check_launch("k_col(2d y)");
+  }
+  XInvArgs<T> xinv_args(void* ws) const {
+    XInvArgs<T> a{};
+    a.in = A(ws);
+    a.in_fs = (long long)ny * mxp;
+    a.in_ss = 8ll * ny * mxp;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.fs = ncell;
+    a.ss = 4 * ncell;
+    a.gamma = T(gamma);
+    a.rho_min = T(1e-8);
+    a.p_min = T(1e-10);
+    a.tw = dx.tw;
+    return a;
+  }
+  void deriv(int axis, int nb, const T* in, T* out, const int* active, void* ws, hipStream_t st) const {
+    xfwd(XF_COPY, in, nb, axis == 0 ? 0xFu : 0u, ws, st);
+    ypass(false, axis == 1, nb, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = 4;
+    a.out = out;
+    a.active = active;
+    launch_xinv<T, 4, XI_OUT>(nx, ny, nb, a, st);
+    check_launch("k_xinv(2d)");
+  }
+  void rhs(int nb, const T* U, T* out, void* ws, hipStream_t st) const {
+    xfwd(XF_FLUX2, U, nb, 0xFu, ws, st);
+    ypass(true, 0, nb, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = 4;
+    a.out = out;
+    launch_xinv<T, 4, XI_RHS>(nx, ny, nb, a, st);
+    check_launch("k_xinv(2d rhs)");
+  }
+  // RK4 with clamp (phrike/solver.py:502-513) and the filter when asked
+  // (cadence decided by the caller, phrike/solver.py:645-647)
+  void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active,
+            bool filter, void* ws, hipStream_t st) const {
+    T* us = Us(ws);
+    for (int s = 1; s <= 4; ++s) {
+      xfwd(XF_FLUX2, s == 1 ? U : us, nb, 0xFu, ws, st);
+      ypass(true, 0, nb, ws, st);
+      XInvArgs<T> a = xinv_args(ws);
+      a.nf = 4;
+      a.out = out;
+      a.un = U;
+      a.us = us;
+      a.stage = s;
+      a.dt = dt;
+      a.dt_dev = dt_dev;
+      a.active = active;
+      launch_xinv<T, 4, XI_STAGE2>(nx, ny, nb, a, st);
+      check_launch("k_xinv(2d stage)");
+    }
+    if (filter) deriv(-1, nb, out, out, active, ws, st);
+  }
+  void max_speed(int nb, const T* U, double* out, double* dt_out, double cfl, void* ws,
+                 hipStream_t st) const {
+    double* part = Part(ws);
+    const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    k_maxspeed_partial<T, 2><<<dim3(npb, nb), NT, 0, st>>>(U, ncell, 4 * ncell, ncell, T(gamma), part);
+    check_launch("k_maxspeed_partial(2d)");
+    const double h = std::min(lx / nx, ly / ny);
+    k_max_final<2><<<nb, NT, 0, st>>>(part, npb, out, dt_out, cfl * h);
+    check_launch("k_max_final(2d)");
+  }
+  void sums(int nb, const T* U, double* out, void* ws, hipStream_t st) const {
+    double* part = Part(ws);
+    const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    k_sums_partial<T, 2><<<dim3(npb, nb), NT, 0, st>>>(U, ncell, 4 * ncell, ncell, T(gamma), part);
+    check_launch("k_sums_partial(2d)");
+    k_sums_final<4><<<nb, NT, 0, st>>>(part, npb, out);
+    check_launch("k_sums_final(2d)");
+  }
+};
+
+}  // namespace ph

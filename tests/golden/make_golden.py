@@ -1,0 +1,248 @@
+"""Write golden input/output vectors from the reference, in this container only.
+
+Usage:  python tests/golden/make_golden.py [--skip-long]
+
+The reference package at /root/reference is imported by path (its package
+``__init__`` is bypassed so nothing beyond the numerics modules is loaded, and
+no bytecode is written into the reference tree).  Only functions whose exact
+composition the closed-form checks cannot pin are recorded:
+
+* F1  3D RK4 step + post-step filter (SpectralSolver3D.step) on the benchmark
+      TGV at 16^3 and 32^3, dt frozen from compute_dt(U0) at cfl 0.4, after 1
+      and 10 steps (phrike/solver.py:1220-1337);
+* F2  the same at 64^3 for 200 steps: per-field norms/sums, a stride-4
+      subsample and the conserved sums;
+* F3  one _compute_rhs_3d on a band-limited random state at 16^3 and on a
+      non-cubic 16x8x32 grid (phrike/solver.py:1177-1184);
+* F5  LegendreLobattoBasis1D nodes, weights, D_x and nodal filter for
+      N = 8, 16, 32, 128 and checksums for N = 256 (phrike/basis/legendre.py);
+* F8  TGV 256^3, 200 steps, dt frozen at t=0: stride-8 subsample, the
+      Fourier modes |m| <= 8, norms and conserved sums after 1, 3, 200 steps;
+* F6  2D KHI (periodic profile) 10 steps of SpectralSolver2D.step with the
+      per-step compute_dt, clamp and filter, at 64^2 and 32x16, plus a
+      filter-cadence case (phrike/solver.py:428-648).
+
+The outputs are plain ``.npz`` data (no pickles).  numba, pyfftw and finufft
+are absent here, so the reference runs its plain numpy/scipy branches; the
+versions used are recorded in every file.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+import types
+
+import numpy as np
+
+sys.dont_write_bytecode = True
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from phrike_amd import initial_conditions as ics  # noqa: E402
+
+
+def _load_reference():
+    pkg = types.ModuleType("phrike")
+    pkg.__path__ = [os.path.join(REF, "phrike")]
+    sys.modules["phrike"] = pkg
+    import importlib
+
+    grid = importlib.import_module("phrike.grid")
+    eqs = importlib.import_module("phrike.equations")
+    solver = importlib.import_module("phrike.solver")
+    leg = importlib.import_module("phrike.basis.legendre")
+    return grid, eqs, solver, leg
+
+
+def _meta():
+    import scipy
+
+    return dict(numpy=np.__version__, scipy=scipy.__version__, numba="absent",
+                pyfftw="absent", created=time.strftime("%Y-%m-%d"))
+
+
+FILT = {"enabled": True, "p": 16, "alpha": 36.0}
+
+
+def f1_f2(grid_m, eq_m, sol_m, out, skip_long):
+    for n, nsteps_list in ((16, (1, 10)), (32, (1, 10))):
+        g = grid_m.Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                          filter_params=dict(FILT), fft_workers=1)
+        e = eq_m.EulerEquations3D(gamma=1.4)
+        s = sol_m.SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+        U0 = ics.taylor_green_3d(n, n, n)
+        dt = s.compute_dt(U0)
+        res = {"U0": U0, "dt": np.float64(dt)}
+        U = U0
+        for k in range(1, max(nsteps_list) + 1):
+            U = s.step(U, dt)
+            if k in nsteps_list:
+                res[f"U_{k}"] = U
+        np.savez_compressed(os.path.join(out, f"tgv3d_{n}.npz"), **res, **_meta())
+        print("F1", n, dt, flush=True)
+    if skip_long:
+        return
+    n, nsteps = 64, 200
+    g = grid_m.Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                      filter_params=dict(FILT), fft_workers=os.cpu_count())
+    e = eq_m.EulerEquations3D(gamma=1.4)
+    s = sol_m.SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+    U = ics.taylor_green_3d(n, n, n)
+    dt = s.compute_dt(U)
+    t0 = time.time()
+    for k in range(nsteps):
+        U = s.step(U, dt)
+    el = time.time() - t0
+    cq = e.conserved_quantities(U)
+    np.savez_compressed(
+        os.path.join(out, "tgv3d_64_200.npz"),
+        dt=np.float64(dt), steps=np.int64(nsteps),
+        sub4=U[:, ::4, ::4, ::4].copy(),
+        l2=np.sqrt((U ** 2).reshape(5, -1).sum(axis=1)),
+        amax=np.abs(U).reshape(5, -1).max(axis=1),
+        sums=U.reshape(5, -1).sum(axis=1),
+        conserved=np.array([cq["mass"], cq["momentum_x"], cq["momentum_y"],
+                            cq["momentum_z"], cq["energy"]]),
+        ref_seconds=np.float64(el), **_meta())
+    print("F2", dt, el, flush=True)
+
+
+def lowmodes(U, m=8):
+    """Fourier coefficients |mx|,|my|,|mz| <= m of each field (rfftn order)."""
+    F = np.fft.rfftn(U, axes=(-3, -2, -1))
+    idx = np.r_[0:m + 1, -m:0]
+    return F[:, idx][:, :, idx][:, :, :, : m + 1]
+
+
+def f8(grid_m, eq_m, sol_m, out, n=256, nsteps=200):
+    """TGV n^3, dt frozen at t=0, nsteps reference steps: stride-8 subsample,
+    low-mode spectrum, norms and conserved sums after 1, 3 and nsteps steps."""
+    g = grid_m.Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                      filter_params=dict(FILT), fft_workers=int(os.environ.get("GOLDEN_WORKERS", "6")))
+    e = eq_m.EulerEquations3D(gamma=1.4)
+    s = sol_m.SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+    U = ics.taylor_green_3d(n, n, n)
+    t0 = time.time()
+    dt = s.compute_dt(U)
+    print("F8 dt", dt, time.time() - t0, flush=True)
+    res = {"dt": np.float64(dt), "steps": np.int64(nsteps)}
+    t0 = time.time()
+    for k in range(1, nsteps + 1):
+        U = s.step(U, dt)
+        if k in (1, 3, nsteps):
+            cq = e.conserved_quantities(U)
+            res[f"sub8_{k}"] = U[:, ::8, ::8, ::8].copy()
+            res[f"low8_{k}"] = lowmodes(U)
+            res[f"l2_{k}"] = np.sqrt((U ** 2).reshape(5, -1).sum(axis=1))
+            res[f"sums_{k}"] = U.reshape(5, -1).sum(axis=1)
+            res[f"conserved_{k}"] = np.array([cq["mass"], cq["momentum_x"], cq["momentum_y"],
+                                              cq["momentum_z"], cq["energy"]])
+            np.savez_compressed(os.path.join(out, f"tgv3d_{n}_{nsteps}.npz"), **res,
+                                ref_seconds=np.float64(time.time() - t0), **_meta())
+        if k % 10 == 0:
+            print("F8 step", k, time.time() - t0, flush=True)
+
+
+def f3(grid_m, eq_m, sol_m, out):
+    res = {}
+    for tag, (nx, ny, nz) in (("cube16", (16, 16, 16)), ("box16x8x32", (16, 8, 32))):
+        g = grid_m.Grid3D(nx, ny, nz, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                          filter_params=dict(FILT), fft_workers=1)
+        e = eq_m.EulerEquations3D(gamma=1.4)
+        U = ics.smooth_random_state_3d(nx, ny, nz, seed=11, kmax=4)
+        res[f"{tag}_U"] = U
+        res[f"{tag}_rhs"] = sol_m._compute_rhs_3d(g, e, U)
+        res[f"{tag}_dx_rho"] = g.dx1(U[0])
+        res[f"{tag}_dy_rho"] = g.dy1(U[0])
+        res[f"{tag}_dz_rho"] = g.dz1(U[0])
+        res[f"{tag}_filter"] = g.apply_spectral_filter(U)
+        if tag == "cube16":
+            Fx, Fy, Fz = e.flux(U)
+            res[f"{tag}_Fx"], res[f"{tag}_Fy"], res[f"{tag}_Fz"] = Fx, Fy, Fz
+        res[f"{tag}_maxc"] = np.float64(e.max_wave_speed(U))
+        s = sol_m.SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+        dt = s.compute_dt(U)
+        res[f"{tag}_dt"] = np.float64(dt)
+        res[f"{tag}_step"] = s.step(U, dt)
+    # filter disabled, dealias on: the post-step filter still applies the mask
+    g = grid_m.Grid3D(16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                      filter_params={"enabled": False}, fft_workers=1)
+    e = eq_m.EulerEquations3D(gamma=1.4)
+    s = sol_m.SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+    U = res["cube16_U"]
+    res["nofilter_step"] = s.step(U, 1e-3)
+    np.savez_compressed(os.path.join(out, "rhs3d.npz"), **res, **_meta())
+    print("F3", flush=True)
+
+
+def f5(leg_m, out):
+    res = {}
+    for n in (8, 16, 32, 128, 256):
+        L = 1.0
+        b = leg_m.LegendreLobattoBasis1D(n, L)
+        D = np.asarray(b._D_x)
+        if n <= 128:
+            res[f"N{n}_x"] = np.asarray(b.nodes())
+            res[f"N{n}_w"] = np.asarray(b.quadrature_weights())
+            res[f"N{n}_D"] = D
+            res[f"N{n}_filter_p16_a36"] = np.asarray(b._build_nodal_filter_np(16, 36.0))
+        res[f"N{n}_rowsum_absmax"] = np.float64(np.abs(D.sum(axis=1)).max())
+        res[f"N{n}_fro"] = np.float64(np.linalg.norm(D))
+        res[f"N{n}_x_sum"] = np.float64(np.asarray(b.nodes()).sum())
+        res[f"N{n}_D_sum_abs"] = np.float64(np.abs(D).sum())
+        res[f"N{n}_D_diag"] = np.diag(D).copy()
+    np.savez_compressed(os.path.join(out, "lgl.npz"), **res, **_meta())
+    print("F5", flush=True)
+
+
+def f6(grid_m, eq_m, sol_m, out):
+    res = {}
+    cases = (("sq64", 64, 64, 1), ("rect32x16", 32, 16, 1), ("cad2_32", 32, 32, 2))
+    for tag, nx, ny, interval in cases:
+        fp = dict(FILT)
+        fp["interval"] = interval
+        g = grid_m.Grid2D(nx, ny, 1.0, 1.0, dealias=True, filter_params=fp, fft_workers=1,
+                          basis_x="fourier", basis_y="fourier")
+        e = eq_m.EulerEquations2D(gamma=1.4)
+        s = sol_m.SpectralSolver2D(g, e, scheme="rk4", cfl=0.4)
+        U = ics.khi_2d_periodic(nx, ny, seed=5)
+        res[f"{tag}_U0"] = U
+        dts = []
+        for k in range(10):
+            dt = s.compute_dt(U)
+            dts.append(dt)
+            U = s.step(U, dt)
+            if k == 0:
+                res[f"{tag}_U1"] = U
+        res[f"{tag}_U10"] = U
+        res[f"{tag}_dts"] = np.array(dts)
+        res[f"{tag}_rhs0"] = sol_m._compute_rhs_2d(g, e, res[f"{tag}_U0"])
+    np.savez_compressed(os.path.join(out, "khi2d.npz"), **res, **_meta())
+    print("F6", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-long", action="store_true")
+    ap.add_argument("--only", default="")
+    a = ap.parse_args()
+    grid_m, eq_m, sol_m, leg_m = _load_reference()
+    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6"}
+    if "f3" in only:
+        f3(grid_m, eq_m, sol_m, HERE)
+    if "f5" in only:
+        f5(leg_m, HERE)
+    if "f6" in only:
+        f6(grid_m, eq_m, sol_m, HERE)
+    if "f1" in only:
+        f1_f2(grid_m, eq_m, sol_m, HERE, a.skip_long)
+    if "f8" in only:
+        f8(grid_m, eq_m, sol_m, HERE)
+
+
+if __name__ == "__main__":
+    main()
