@@ -1,0 +1,184 @@
+"""Benchmark of the hip backend's 3D RK4 step (one GPU).
+
+Workload (default): 3D Taylor-Green vortex, 512^3, FP64, Fourier basis on
+[0, 2pi)^3, gamma = 1.4, 2/3 dealiasing, exponential filter alpha = 36,
+order 16, RK4 with the post-step filter, dt from CFL 0.4 frozen at t = 0.
+A step is one RK4 step of the whole grid (4 fused RHS evaluations, stage
+combines, post-step filter).  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line:
+  value      Mcell-updates/s = N^3 * steps / wall time
+  roofline   the dominant kernel's algorithmic bytes per launch over its
+             average launch duration (HIP events in the timed region)
+             against 8 TB/s; plus the whole step against the pencil-pipeline
+             byte model C(N) s N^3 (hbm_frac_step)
+  cpu_baseline  the oracle (numpy/scipy restatement of the reference's
+             NumPy path) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "3D Euler RK4 Mcell-updates/s + %HBM peak; FP32 vs FP64 err; ensemble p99"
+
+
+def byte_model(n: int):
+    """Per-launch algorithmic bytes of each pipeline kernel in units of
+    s*N^3 (s = element size), and the per-step total C(N)."""
+    k = n // 3
+    a = 2 * (k + 1) / n
+    b = 2 * (k + 1) * (2 * k + 1) / n ** 2
+    per = {
+        "xfwd_flux": 5 + 15 * a, "yfwd_div": 15 * a + 10 * b, "z_div": 15 * b,
+        "yinv": 5 * b + 5 * a, "xinv_stage1": 5 * a + 15, "xinv_stage23": 5 * a + 20,
+        "xinv_stage4": 5 * a + 15, "xfwd_copy": 5 + 5 * a, "yfwd_copy": 5 * a + 5 * b,
+        "z_copy": 10 * b, "xinv_out": 5 * a + 5,
+    }
+    C = 4 * (5 + 40 * a + 30 * b) + 70 + 10 + 20 * a + 20 * b
+    return per, C
+
+
+def cpu_baseline(n: int, steps: int):
+    """Time the oracle's step on the host: `steps` steps of TGV n^3 FP64."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import taylor_green_3d
+
+    cores = min(16, sn.default_workers())
+    g = sn.Grid3DOracle(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                        filter_params={"enabled": True, "p": 16, "alpha": 36.0}, fft_workers=cores)
+    s = sn.Solver3DOracle(g, sn.Euler3DOracle(1.4), cfl=0.4)
+    U = taylor_green_3d(n, n, n)
+    dt = s.compute_dt(U)
+    g.fftn(U[0])  # plan warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        U = s.step(U, dt)
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle (numpy/{sn.FFT_LIBRARY} restatement of the reference NumPy path) "
+                      f"TGV {n}^3 FP64, {steps} RK4 step(s) incl. post-step filter, "
+                      f"{cores} FFT workers, {el:.1f} s"}
+
+
+def load_traffic(kernel: str, n: int, prec: str):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    try:
+        d = json.load(open(path))
+        return d.get(f"{kernel}@{n}^3/{prec}")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--precision", choices=["double", "single"], default="double")
+    ap.add_argument("--cpu-size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the 3D pseudo-spectral step is not sharded "
+              "across GPUs in this backend (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from phrike_amd import _lib
+    from phrike_amd.equations import EulerEquations3D
+    from phrike_amd.grid import Grid3D
+    from phrike_amd.initial_conditions import taylor_green_3d_torch
+    from phrike_amd.solver import SpectralSolver3D
+
+    n = args.size
+    dev = torch.device("cuda:0")
+    g = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+               filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip",
+               precision=args.precision)
+    s = SpectralSolver3D(g, EulerEquations3D(1.4), scheme="rk4", cfl=0.4)
+    U = taylor_green_3d_torch(n, dev, g.dtype)
+    out = torch.empty_like(U)
+    dt = s.compute_dt(U)
+    print(f"[bench] TGV {n}^3 {args.precision} dt={dt:.6e} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        s.step_into(U, out, dt)
+        U, out = out, U
+    torch.cuda.synchronize()
+    if not args.no_kernel_timing:
+        _lib.timing_collect()
+        _lib.timing_enable(True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        s.step_into(U, out, dt)
+        U, out = out, U
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    kt = {}
+    if not args.no_kernel_timing:
+        _lib.timing_enable(False)
+        kt = _lib.timing_collect()
+    ms_per_step = ev_ms / args.steps
+    value = n ** 3 * args.steps / (ev_ms / 1e3) / 1e6
+    es = 8 if args.precision == "double" else 4
+    per, C = byte_model(n)
+    step_bytes = C * es * n ** 3
+    kernels = {}
+    dominant = None
+    for tag, (cnt, ms) in kt.items():
+        avg_ms = ms / cnt
+        alg = per.get(tag, 0.0) * es * n ** 3
+        kernels[tag] = {"launches": cnt, "avg_ms": round(avg_ms, 4), "total_ms": round(ms, 3),
+                        "alg_GBs": round(alg / (avg_ms / 1e3) / 1e9, 1) if alg else None}
+        if dominant is None or ms > kt[dominant][1]:
+            dominant = tag
+    roof = None
+    if dominant is not None:
+        cnt, ms = kt[dominant]
+        alg = per[dominant] * es * n ** 3
+        ach = alg / (ms / cnt / 1e3)
+        roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK, 4), "traffic": load_traffic(dominant, n, args.precision),
+                "kernel": dominant, "alg_bytes_per_launch": int(alg)}
+    sums = s.conserved_sums(U)
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] cpu baseline: oracle TGV {args.cpu_size}^3 x {args.cpu_steps}", file=sys.stderr,
+              flush=True)
+        cpu = cpu_baseline(args.cpu_size, args.cpu_steps)
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "double" else "f32", "data": "synthetic",
+        "config": {"workload": f"TGV {n}^3 Fourier pseudo-spectral Euler, RK4 + post-step filter "
+                               f"(2/3 dealias, exp filter alpha=36 p=16), dt frozen from CFL 0.4",
+                   "grid": [n, n, n], "parallelism": "single GPU"},
+        "hbm_frac_step": round(step_bytes / (ms_per_step / 1e3) / HBM_PEAK, 4),
+        "step_model_bytes": int(step_bytes), "wall_s": round(wall, 3),
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
+        "mass_drift": float(sums[0] / n ** 3 - 1.0),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -45,6 +45,13 @@ typedef struct ph_plan2d_s* ph_plan2d_t;
 /* Library ABI version (no reference counterpart). */
 int ph_abi_version(void);
 
+/* Per-kernel timing (no reference counterpart; measurement only).  While
+ * enabled, every kernel launch of the library is bracketed by HIP events on
+ * its stream.  ph_timing_collect() waits for the recorded events and writes
+ * lines "<kernel-tag> <launches> <total-ms>\n" into buf, then resets. */
+int ph_timing_enable(int on);
+int ph_timing_collect(char* buf, size_t len);
+
 /* Message for the last failed call on this thread.  Replaces the reference's
  * Python exceptions (ValueError/RuntimeError, phrike/grid.py:66-84, 308). */
 const char* ph_last_error(void);
@@ -139,6 +146,25 @@ int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max
  * (phrike/solver.py:734-757). */
 int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out4_dev, void* ws,
                         void* stream);
+
+/* ------------------------------------------------------------------------
+ * Dense LGL collocation derivative on matrix cores:
+ *   out[b, i, x] = sum_j D[i, j] F[b, j, x]      (b < nbatch)
+ * and, when add != NULL, the hybrid-RHS epilogue out = -(add + D.F).
+ * F, add, out are (nbatch, ny, nx); D/Dhi/Dlo are row-major ny x ny.
+ * mode PH_GLL_F64: FP64 MFMA with D (float64); data float64.
+ * mode PH_GLL_SPLIT_F32: FP32 MFMA, D split as Dhi + Dlo (float32 each,
+ *   Dhi = fp32(D), Dlo = fp32(D - Dhi)); data float32.
+ * mode PH_GLL_SPLIT_F64IO: FP32 MFMA on float64 data split as hi + lo:
+ *   Dhi*Fhi + Dhi*Flo + Dlo*Fhi, result in float64.
+ * ny, nx multiples of 16.  Replaces LegendreLobattoBasis2D.dy /
+ * LegendreLobattoBasis1D.dx applies (phrike/basis/legendre.py:787-802,
+ * 1020-1028; torch matmul at 1027). */
+#define PH_GLL_F64 0
+#define PH_GLL_SPLIT_F32 1
+#define PH_GLL_SPLIT_F64IO 2
+int ph_gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
+                 const void* F, const void* add, void* out, void* stream);
 
 #ifdef __cplusplus
 }

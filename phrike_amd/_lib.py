@@ -10,7 +10,8 @@ import ctypes
 import os
 from ctypes import POINTER, c_double, c_int, c_size_t, c_void_p
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libphrike_hip.so")
+LIB_PATH = os.environ.get("PHRIKE_HIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                           "libphrike_hip.so")
 
 PH_F64 = 0
 PH_F32 = 1
@@ -19,6 +20,8 @@ PH_F32 = 1
 _SIGNATURES = {
     "ph_abi_version": (c_int, []),
     "ph_last_error": (ctypes.c_char_p, []),
+    "ph_timing_enable": (c_int, [c_int]),
+    "ph_timing_collect": (c_int, [ctypes.c_char_p, c_size_t]),
     "ph_plan3d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_double, c_double,
                                  c_int, c_int, c_int, c_int, c_double, c_double]),
     "ph_plan3d_destroy": (c_int, [c_void_p]),
@@ -39,7 +42,13 @@ _SIGNATURES = {
     "ph_max_wave_speed2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                     c_void_p]),
     "ph_conserved_sums2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_gll_apply": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p]),
 }
+
+PH_GLL_F64 = 0
+PH_GLL_SPLIT_F32 = 1
+PH_GLL_SPLIT_F64IO = 2
 
 _lib = None
 
@@ -64,6 +73,21 @@ def load(path: str = LIB_PATH):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def timing_enable(on: bool) -> None:
+    call("ph_timing_enable", int(bool(on)))
+
+
+def timing_collect() -> dict:
+    """{kernel tag: (launches, total ms)} since the last collect."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    call("ph_timing_collect", buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        tag, n, ms = line.split()
+        out[tag] = (int(n), float(ms))
+    return out
 
 
 def symbols():

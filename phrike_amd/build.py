@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphrike_hip.so")
-SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip"]
+SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "gll.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PH_OFFLOAD_ARCH", "gfx950")
@@ -41,22 +41,22 @@ def _compile(src: str, outdir: str, extra) -> str:
     return obj
 
 
-def build(verbose: bool = True, extra=None) -> str:
+def build(verbose: bool = True, extra=None, lib: str = LIB, outdir: str = None) -> str:
     extra = list(extra or [])
-    outdir = os.path.join(HERE, "_build")
+    outdir = outdir or os.path.join(HERE, "_build")
     os.makedirs(outdir, exist_ok=True)
     jobs = min(len(SOURCES), max(1, os.cpu_count() or 1))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, outdir, extra), SOURCES))
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":

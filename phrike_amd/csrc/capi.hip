@@ -12,6 +12,8 @@ extern template struct Plan3D<double>;
 extern template struct Plan3D<float>;
 extern template struct Plan2D<double>;
 extern template struct Plan2D<float>;
+void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
+               const void* F, const void* add, void* out, hipStream_t st);
 }  // namespace ph
 
 namespace {
@@ -66,6 +68,28 @@ hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 extern "C" {
 
 int ph_abi_version(void) { return 1; }
+
+int ph_timing_enable(int on) {
+  return guard([&] {
+    ph::KTimer& k = ph::ktimer();
+    if (!on) k.collect();
+    k.on = on != 0;
+  });
+}
+
+int ph_timing_collect(char* buf, size_t len) {
+  return guard([&] {
+    need(buf != nullptr && len > 0, "null buffer");
+    ph::KTimer& k = ph::ktimer();
+    k.collect();
+    std::string out;
+    for (auto& kv : k.acc)
+      out += kv.first + " " + std::to_string(kv.second.second) + " " + std::to_string(kv.second.first) + "\n";
+    k.acc.clear();
+    std::strncpy(buf, out.c_str(), len - 1);
+    buf[len - 1] = 0;
+  });
+}
 
 const char* ph_last_error(void) { return g_err.c_str(); }
 
@@ -247,6 +271,11 @@ int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out
       p.sums(nbatch, static_cast<const R*>(U), out4_dev, ws, S(stream));
     });
   });
+}
+
+int ph_gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
+                 const void* F, const void* add, void* out, void* stream) {
+  return guard([&] { ph::gll_apply(mode, ny, nx, nbatch, D, Dhi, Dlo, F, add, out, S(stream)); });
 }
 
 }  // extern "C"

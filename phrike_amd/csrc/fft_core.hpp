@@ -205,6 +205,67 @@ __device__ __forceinline__ void fft_regs(cx<T>* buf, const cx<T>* tw, int tid,
   __syncthreads();
 }
 
+// As fft_regs, but the first pass takes its inputs from ld(line, position)
+// (global memory, zero-fill, ...) instead of LDS, which saves the staging
+// write/read and one barrier per transform.  The caller's previous use of buf
+// must have ended with a barrier (fft_regs and fft_regs_g end with one).
+template <typename T, int N, int L, int DIR, int NT, class Ld>
+__device__ __forceinline__ void fft_regs_g(cx<T>* buf, const cx<T>* tw, int tid,
+                                           cx<T> (&v)[LastRegs<N, L, NT>::PER][LastRegs<N, L, NT>::R],
+                                           Ld ld) {
+  constexpr int NP = Plan<N>::npass;
+  constexpr int R0 = Plan<N>::radix(0);
+  constexpr int NB0 = L * (N / R0);
+  if constexpr (NP == 1) {
+    constexpr int PER = LastRegs<N, L, NT>::PER;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int b = tid + q * NT;
+      if (NB0 % NT == 0 || b < NB0) {
+        const int l = b % L, j = b / L;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[q][r] = ld(l, j + r * (N / R0));
+        dftR<R0, DIR>(v[q]);
+      }
+    }
+  } else {
+    constexpr int PER0 = (NB0 + NT - 1) / NT;
+    cx<T> w[PER0][R0];
+#pragma unroll
+    for (int q = 0; q < PER0; ++q) {
+      const int b = tid + q * NT;
+      if (NB0 % NT == 0 || b < NB0) {
+        const int l = b % L, j = b / L;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) w[q][r] = ld(l, j + r * (N / R0));
+        dftR<R0, DIR>(w[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PER0; ++q) {
+      const int b = tid + q * NT;
+      if (NB0 % NT == 0 || b < NB0) {
+        const int l = b % L, j = b / L;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) buf[(j * R0 + r) * L + l] = w[q][r];
+      }
+    }
+    __syncthreads();
+    constexpr int LP = NP - 1;
+    constexpr int R = Plan<N>::radix(LP);
+    constexpr int NS = PassNs<N, LP>::value;
+    constexpr int NB = L * (N / R);
+    constexpr int PER = LastRegs<N, L, NT>::PER;
+    passes_lds_range<T, N, L, DIR, NT, 1, LP>(buf, tw, tid);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int b = tid + q * NT;
+      if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
+    }
+    __syncthreads();
+  }
+}
+
 // Is register slot q valid for this thread, and which (line, position) is
 // output r of it?
 template <int N, int L, int NT>

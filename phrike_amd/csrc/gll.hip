@@ -1,0 +1,148 @@
+// Dense Legendre-Gauss-Lobatto collocation derivative on matrix cores:
+//   out[b, i, x] = sum_j D[i, j] * F[b, j, x]     (b over fields x runs)
+// i.e. D (ny x ny) applied along the second-to-last axis of a (nb, ny, nx)
+// stack, as LegendreLobattoBasis2D.dy / LegendreLobattoBasis1D.dx do with a
+// tensordot/matmul (phrike/basis/legendre.py:787-802, 1020-1028).
+//
+// One wave64 owns a 16 x 16 output tile and walks K = ny in steps of 4 with
+// v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (split FP32).
+// Operand layout of both instructions (wave64): lane l holds A[l%16][l/16],
+// B[l/16][l%16], and the accumulator register v of lane l is
+// D[4*(l/16)+v][l%16].
+//
+// Split precision: D = Dhi + Dlo with Dhi = fp32(D), Dlo = fp32(D - Dhi)
+// (built on the host from the FP64 table).  For FP32 data the product is
+// Dhi*F + Dlo*F; for FP64 data F = Fhi + Flo as well and the product is
+// Dhi*Fhi + Dhi*Flo + Dlo*Fhi, each term accumulated in its own FP32
+// register set and summed in FP64 at the end.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/phrike_hip.h"
+#include "pipeline.hpp"
+
+namespace ph {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int GLL_WAVES = 4;  // a block: 4 waves stacked along i (64 rows)
+
+// EPI: 0 out = acc; 1 out = -(acc + add)  (hybrid RHS epilogue)
+template <int EPI>
+__global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_f64(const double* __restrict__ D,
+                                                           const double* __restrict__ F,
+                                                           const double* __restrict__ add,
+                                                           double* __restrict__ out, int ny, int nx) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int i0 = (blockIdx.y * GLL_WAVES + wave) * 16;
+  const int x0 = blockIdx.x * 16;
+  const long long bo = (long long)blockIdx.z * ny * nx;
+  if (i0 >= ny) return;
+  const int r = lane & 15, q = lane >> 4;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  const double* Drow = D + (long long)(i0 + r) * ny;
+  const double* Fc = F + bo + x0 + r;
+  for (int k0 = 0; k0 < ny; k0 += 4) {
+    const double a = Drow[k0 + q];
+    const double b = Fc[(long long)(k0 + q) * nx];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const long long o = bo + (long long)(i0 + 4 * q + v) * nx + x0 + r;
+    double val = acc[v];
+    if constexpr (EPI == 1) val = -(add[o] + val);
+    out[o] = val;
+  }
+}
+
+// split-precision FP32; FP32 data (T = float) or FP64 data (T = double)
+template <typename T, int EPI>
+__global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_split(const float* __restrict__ Dhi,
+                                                             const float* __restrict__ Dlo,
+                                                             const T* __restrict__ F,
+                                                             const T* __restrict__ add,
+                                                             T* __restrict__ out, int ny, int nx) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int i0 = (blockIdx.y * GLL_WAVES + wave) * 16;
+  const int x0 = blockIdx.x * 16;
+  const long long bo = (long long)blockIdx.z * ny * nx;
+  if (i0 >= ny) return;
+  const int r = lane & 15, q = lane >> 4;
+  f4 c1 = {0.f, 0.f, 0.f, 0.f}, c2 = {0.f, 0.f, 0.f, 0.f}, c3 = {0.f, 0.f, 0.f, 0.f};
+  const long long drow = (long long)(i0 + r) * ny;
+  const T* Fc = F + bo + x0 + r;
+  for (int k0 = 0; k0 < ny; k0 += 4) {
+    const float ah = Dhi[drow + k0 + q];
+    const float al = Dlo[drow + k0 + q];
+    const T bv = Fc[(long long)(k0 + q) * nx];
+    const float bh = float(bv);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, c2, 0, 0, 0);
+    if constexpr (sizeof(T) == 8) {
+      const float bl = float(bv - double(bh));
+      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, c3, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const long long o = bo + (long long)(i0 + 4 * q + v) * nx + x0 + r;
+    double val = double(c1[v]) + (double(c2[v]) + double(c3[v]));
+    if constexpr (EPI == 1) val = -(double(add[o]) + val);
+    out[o] = T(val);
+  }
+}
+
+}  // namespace ph
+
+namespace ph {
+
+void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
+               const void* F, const void* add, void* out, hipStream_t st) {
+  if (ny < 16 || nx < 16 || ny % 16 || nx % 16 || nbatch < 1 || nbatch > 65535)
+    throw Error(PH_ERR_ARG, "ny, nx must be positive multiples of 16; nbatch in [1, 65535]");
+  if (!F || !out) throw Error(PH_ERR_ARG, "null buffer");
+  dim3 block(64 * GLL_WAVES);
+  dim3 grid(nx / 16, (ny + 16 * GLL_WAVES - 1) / (16 * GLL_WAVES), nbatch);
+  const bool epi = add != nullptr;
+  switch (mode) {
+    case PH_GLL_F64:
+      if (!D) throw Error(PH_ERR_ARG, "null D");
+      if (epi)
+        k_gll_f64<1><<<grid, block, 0, st>>>((const double*)D, (const double*)F, (const double*)add,
+                                             (double*)out, ny, nx);
+      else
+        k_gll_f64<0><<<grid, block, 0, st>>>((const double*)D, (const double*)F, nullptr, (double*)out,
+                                             ny, nx);
+      break;
+    case PH_GLL_SPLIT_F32:
+      if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
+      if (epi)
+        k_gll_split<float, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                      (const float*)F, (const float*)add, (float*)out,
+                                                      ny, nx);
+      else
+        k_gll_split<float, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                      (const float*)F, nullptr, (float*)out, ny, nx);
+      break;
+    case PH_GLL_SPLIT_F64IO:
+      if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
+      if (epi)
+        k_gll_split<double, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                       (const double*)F, (const double*)add,
+                                                       (double*)out, ny, nx);
+      else
+        k_gll_split<double, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                       (const double*)F, nullptr, (double*)out, ny, nx);
+      break;
+    default:
+      throw Error(PH_ERR_ARG, "unknown mode");
+  }
+  check_launch("k_gll");
+}
+
+}  // namespace ph

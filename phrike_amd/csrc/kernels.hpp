@@ -212,38 +212,27 @@ struct ColArgs {
   const T* kk;   // by compact index
 };
 
+#ifndef PH_COL_ELEMS_F64
+#define PH_COL_ELEMS_F64 4096
+#endif
+#ifndef PH_COL_ELEMS_F32
+#define PH_COL_ELEMS_F32 8192
+#endif
+
 template <typename T, int N>
 struct ColTile {
-  static constexpr int ELEMS = (sizeof(T) == 8) ? 4096 : 8192;
+  static constexpr int ELEMS = (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : PH_COL_ELEMS_F32;
   static constexpr int TM = clampi(ELEMS / N, 1, 64);
   // 8 (FP64) or 16 (FP32) complex values of each register array per thread
   static constexpr int NT = clampi((N * TM) / (sizeof(T) == 8 ? 8 : 16), 64, 512);
 };
 
-template <typename T, int N, int TM, int NT>
-__device__ __forceinline__ void col_load_full(cx<T>* buf, const cx<T>* src, long long ls, int c0,
-                                              int ncol, int tid) {
-  for (int i = tid; i < N * TM; i += NT) {
-    const int n = i / TM, l = i - n * TM;
-    const int col = c0 + l;
-    buf[i] = (col < ncol) ? src[(long long)n * ls + col] : mk(T(0), T(0));
-  }
-}
-
-template <typename T, int N, int TM, int NT>
-__device__ __forceinline__ void col_load_compact(cx<T>* buf, const cx<T>* src, long long ls, int c0,
-                                                 int ncol, const Keep& kp, int tid) {
-  for (int i = tid; i < N * TM; i += NT) {
-    const int n = i / TM, l = i - n * TM;
-    const int col = c0 + l;
-    cx<T> v = mk(T(0), T(0));
-    if (col < ncol && kp.kept(n, N)) v = src[(long long)kp.compact(n, N) * ls + col];
-    buf[i] = v;
-  }
-}
+#ifndef PH_COL_WAVES_PER_EU
+#define PH_COL_WAVES_PER_EU 2
+#endif
 
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((ColTile<T, N>::NT)) void k_col(ColArgs<T> a) {
+__global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
   constexpr int TM = ColTile<T, N>::TM;
   constexpr int NT = ColTile<T, N>::NT;
   using LR = LastRegs<N, TM, NT>;
@@ -258,21 +247,26 @@ __global__ __launch_bounds__((ColTile<T, N>::NT)) void k_col(ColArgs<T> a) {
   const Keep kp = a.keep;
   cx<T> X[PER][R], Z[PER][R];
 
+  const long long ls = a.in_ls;
+  const int ncol = a.ncol;
+  // element (line l, position n) of field f's tile, zero outside the grid
+  auto full = [&](int f) {
+    const cx<T>* src = in + (long long)f * a.in_fs + c0;
+    return [=](int l, int n) { return (c0 + l < ncol) ? src[(long long)n * ls + l] : mk(T(0), T(0)); };
+  };
+  __syncthreads();  // twiddles
 #pragma unroll 1
   for (int c = 0; c < a.nc; ++c) {
     if constexpr (MODE == CM_INV) {
-      col_load_compact<T, N, TM, NT>(buf, in + (long long)c * a.in_fs, a.in_ls, c0, a.ncol, kp, tid);
-      __syncthreads();
-      fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
+      const cx<T>* src = in + (long long)c * a.in_fs + c0;
+      fft_regs_g<T, N, TM, +1, NT>(buf, tw, tid, X, [=](int l, int n) {
+        return (c0 + l < ncol && kp.kept(n, N)) ? src[(long long)kp.compact(n, N) * ls + l] : mk(T(0), T(0));
+      });
     } else {
-      col_load_full<T, N, TM, NT>(buf, in + (long long)c * a.in_fs, a.in_ls, c0, a.ncol, tid);
-      __syncthreads();
-      fft_regs<T, N, TM, -1, NT>(buf, tw, tid, X);
+      fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, X, full(c));
       if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) {
         const int second = (MODE == CM_FWD_DIV3) ? 5 + c : a.nc + c;
-        col_load_full<T, N, TM, NT>(buf, in + (long long)second * a.in_fs, a.in_ls, c0, a.ncol, tid);
-        __syncthreads();
-        fft_regs<T, N, TM, -1, NT>(buf, tw, tid, Z);
+        fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, Z, full(second));
       }
     }
     // combine on the forward outputs (all modes but CM_INV)
@@ -315,9 +309,7 @@ __global__ __launch_bounds__((ColTile<T, N>::NT)) void k_col(ColArgs<T> a) {
       }
       if constexpr (MODE == CM_FWD_DIV3) {
         // sigma_y * Fz^ (second output group)
-        col_load_full<T, N, TM, NT>(buf, in + (long long)(10 + c) * a.in_fs, a.in_ls, c0, a.ncol, tid);
-        __syncthreads();
-        fft_regs<T, N, TM, -1, NT>(buf, tw, tid, Z);
+        fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, Z, full(10 + c));
         cx<T>* o2 = out + (long long)(5 + c) * a.out_fs;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {

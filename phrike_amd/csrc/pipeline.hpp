@@ -4,6 +4,8 @@
 
 #include <cmath>
 #include <cstddef>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -121,7 +123,7 @@ inline bool supported_len(int n) { return n >= 8 && n <= 1024 && is_pow2(n); }
 template <typename T, int MODE>
 void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t st) {
   dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, (k_xfwd<T, N_, MODE><<<grid, XThreads<T, N_>::value, 0, st>>>(a)));
+  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xfwd<T, N_, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
 }
 
 template <typename T, int MODE>
@@ -129,17 +131,80 @@ void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
   PH_SWITCH_N(n, ({
     constexpr int TM = ColTile<T, N_>::TM;
     dim3 grid((a.ncol + TM - 1) / TM, nblocky);
-    k_col<T, N_, MODE><<<grid, ColTile<T, N_>::NT, 0, st>>>(a);
+    hipLaunchKernelGGL((k_col<T, N_, MODE>), grid, dim3(ColTile<T, N_>::NT), 0, st, a);
   }));
 }
 
 template <typename T, int NC, int MODE>
 void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t st) {
   dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, (k_xinv<T, N_, NC, MODE><<<grid, XThreads<T, N_>::value, 0, st>>>(a)));
+  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xinv<T, N_, NC, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
 }
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+// Optional per-kernel timing: when enabled, every pipeline launch is
+// bracketed by two HIP events on its own stream; collect() waits for them and
+// accumulates elapsed milliseconds per kernel tag.
+struct KTimer {
+  struct Rec {
+    std::string tag;
+    hipEvent_t a, b;
+  };
+  bool on = false;
+  std::mutex mu;
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> acc;
+
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void collect() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& r : pending) {
+      hip_check(hipEventSynchronize(r.b), "hipEventSynchronize");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+      auto& v = acc[r.tag];
+      v.first += ms;
+      v.second += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+};
+inline KTimer& ktimer() {
+  static KTimer t;
+  return t;
+}
+struct KScope {
+  hipEvent_t a = nullptr, b = nullptr;
+  const char* tag;
+  hipStream_t st;
+  KScope(const char* t, hipStream_t s) : tag(t), st(s) {
+    KTimer& k = ktimer();
+    if (!k.on) return;
+    std::lock_guard<std::mutex> g(k.mu);
+    a = k.get();
+    b = k.get();
+    hip_check(hipEventRecord(a, st), "hipEventRecord");
+  }
+  ~KScope() {
+    if (!a) return;
+    KTimer& k = ktimer();
+    std::lock_guard<std::mutex> g(k.mu);
+    if (hipEventRecord(b, st) == hipSuccess) k.pending.push_back({tag, a, b});
+  }
+};
 
 // ---------------------------------------------------------------------------
 // 3D plan
@@ -220,6 +285,7 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
+    KScope ks(mode == XF_FLUX3 ? "xfwd_flux" : "xfwd_copy", st);
     if (mode == XF_FLUX3) launch_xfwd<T, XF_FLUX3>(nx, ny, nz, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
     check_launch("k_xfwd");
@@ -242,6 +308,7 @@ struct Plan3D : PlanBase {
     a.tw = dy.tw;
     a.sig = dy.sig;
     a.kk = dy.k;
+    KScope ks(div ? "yfwd_div" : "yfwd_copy", st);
     if (div) launch_col<T, CM_FWD_DIV3>(ny, nz, a, st);
     else launch_col<T, CM_FWD_COPY>(ny, nz, a, st);
     check_launch("k_col(y fwd)");
@@ -264,6 +331,7 @@ struct Plan3D : PlanBase {
     a.tw = dz.tw;
     a.sig = dz.sig;
     a.kk = dz.k;
+    KScope ks(div ? "z_div" : "z_copy", st);
     if (div) launch_col<T, CM_FWDINV_DIV>(nz, ay.nr, a, st);
     else launch_col<T, CM_FWDINV_COPY>(nz, ay.nr, a, st);
     check_launch("k_col(z)");
@@ -286,6 +354,7 @@ struct Plan3D : PlanBase {
     a.tw = dy.tw;
     a.sig = dy.sig;
     a.kk = dy.k;
+    KScope ks("yinv", st);
     launch_col<T, CM_INV>(ny, nz, a, st);
     check_launch("k_col(y inv)");
   }
@@ -316,6 +385,7 @@ struct Plan3D : PlanBase {
     XInvArgs<T> a = xinv_args(ws);
     a.nf = nf;
     a.out = out;
+    KScope ks("xinv_out", st);
     launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
     check_launch("k_xinv");
   }
@@ -347,6 +417,7 @@ struct Plan3D : PlanBase {
       a.stage = s;
       a.dt = dt;
       a.dt_dev = dt_dev;
+      KScope ks(s == 4 ? "xinv_stage4" : (s == 1 ? "xinv_stage1" : "xinv_stage23"), st);
       launch_xinv<T, 5, XI_STAGE3>(nx, ny, nz, a, st);
       check_launch("k_xinv(stage)");
     }
@@ -355,18 +426,18 @@ struct Plan3D : PlanBase {
   void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    k_maxspeed_partial<T, 3><<<dim3(nb, 1), NT, 0, st>>>(U, ncell, 0, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_maxspeed_partial<T, 3>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma), part);
     check_launch("k_maxspeed_partial");
     const double h = std::min(lx / nx, std::min(ly / ny, lz / nz));
-    k_max_final<3><<<1, NT, 0, st>>>(part, nb, out, dt_out, cfl * h);
+    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out, dt_out, cfl * h);
     check_launch("k_max_final");
   }
   void sums(const T* U, double* out, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    k_sums_partial<T, 3><<<dim3(nb, 1), NT, 0, st>>>(U, ncell, 0, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_sums_partial<T, 3>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma), part);
     check_launch("k_sums_partial");
-    k_sums_final<5><<<1, NT, 0, st>>>(part, nb, out);
+    hipLaunchKernelGGL((k_sums_final<5>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out);
     check_launch("k_sums_final");
   }
 };
@@ -520,18 +591,18 @@ struct Plan2D : PlanBase {
                  hipStream_t st) const {
     double* part = Part(ws);
     const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    k_maxspeed_partial<T, 2><<<dim3(npb, nb), NT, 0, st>>>(U, ncell, 4 * ncell, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_maxspeed_partial<T, 2>), dim3(npb, nb), dim3(NT), 0, st, U, ncell, 4 * ncell, ncell, T(gamma), part);
     check_launch("k_maxspeed_partial(2d)");
     const double h = std::min(lx / nx, ly / ny);
-    k_max_final<2><<<nb, NT, 0, st>>>(part, npb, out, dt_out, cfl * h);
+    hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out, dt_out, cfl * h);
     check_launch("k_max_final(2d)");
   }
   void sums(int nb, const T* U, double* out, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    k_sums_partial<T, 2><<<dim3(npb, nb), NT, 0, st>>>(U, ncell, 4 * ncell, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_sums_partial<T, 2>), dim3(npb, nb), dim3(NT), 0, st, U, ncell, 4 * ncell, ncell, T(gamma), part);
     check_launch("k_sums_partial(2d)");
-    k_sums_final<4><<<nb, NT, 0, st>>>(part, npb, out);
+    hipLaunchKernelGGL((k_sums_final<4>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out);
     check_launch("k_sums_final(2d)");
   }
 };

@@ -166,3 +166,25 @@ def smooth_random_state_2d(nx: int, ny: int, seed: int = 7, kmax: int = 3,
     ux, uy = 0.5 * field(), 0.5 * field()
     p = 1.0 + 0.2 * field()
     return _conservative2d(rho, ux, uy, p, gamma)
+
+
+def taylor_green_3d_torch(n: int, device, dtype=None, gamma: float = 1.4):
+    """The benchmark TGV of ``taylor_green_3d`` evaluated with torch on the
+    device (same formula; avoids a host meshgrid of n^3 at large n)."""
+    import torch
+
+    dtype = dtype or torch.float64
+    x = torch.arange(n, dtype=torch.float64, device=device) * (2 * np.pi / n)
+    Z = x.view(n, 1, 1)
+    Y = x.view(1, n, 1)
+    X = x.view(1, 1, n)
+    ux = torch.sin(X) * torch.cos(Y) * torch.cos(Z)
+    uy = -torch.cos(X) * torch.sin(Y) * torch.cos(Z)
+    p = 100.0 + (1.0 / 16.0) * (torch.cos(2 * X) + torch.cos(2 * Y)) * (torch.cos(2 * Z) + 2.0)
+    U = torch.empty((5, n, n, n), dtype=torch.float64, device=device)
+    U[0] = 1.0
+    U[1] = ux
+    U[2] = uy
+    U[3] = 0.0
+    U[4] = p / (gamma - 1.0) + 0.5 * (ux * ux + uy * uy)
+    return U.to(dtype)

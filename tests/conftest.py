@@ -67,3 +67,29 @@ def field_errors(a, b, floor=None):
         den = nb if nb > (floor if floor is not None else 1e-8 * scale) else scale
         errs.append(np.linalg.norm(a[f] - b[f]) / den)
     return np.array(errs)
+
+
+def scaled_err(got, ref, scale):
+    """||got - ref|| / max(||ref||, scale): for outputs that can be ~0 by
+    construction (e.g. a derivative of a field whose modes are all masked)."""
+    import numpy as np
+
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return np.linalg.norm(got - ref) / max(np.linalg.norm(ref), float(scale))
+
+
+def group_errors(a, b, groups):
+    """per-field L2 error over the largest reference norm in the field's group
+    (density / momenta / energy): FP32 error on a momentum component that is
+    ~0 (rho*w of the TGV) is measured against the momentum scale."""
+    import numpy as np
+
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    norms = np.array([np.linalg.norm(b[f]) for f in range(b.shape[0])])
+    out = []
+    for f in range(b.shape[0]):
+        g = [grp for grp in groups if f in grp][0]
+        out.append(np.linalg.norm(a[f] - b[f]) / max(norms[list(g)].max(), 1e-300))
+    return np.array(out)

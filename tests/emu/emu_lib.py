@@ -1,0 +1,94 @@
+"""ctypes access to the CPU-emulated backend (tests/emu/build_emu.py) with
+numpy arrays standing in for device buffers."""
+import ctypes
+import os
+
+import numpy as np
+
+from phrike_amd import _lib as _real
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(asan=False):
+    from build_emu import build
+    path = build(asan)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _real._SIGNATURES.items():
+        if name.startswith("ph_gll"):
+            continue
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    return lib
+
+
+def P(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+class Emu3D:
+    def __init__(self, lib, nx, ny, nz, L=(2 * np.pi,) * 3, dtype=np.float64, dealias=True,
+                 filt=(1, 16, 36.0), gamma=1.4):
+        self.lib, self.dtype = lib, dtype
+        self.shape = (nz, ny, nx)
+        h = ctypes.c_void_p()
+        code = 0 if dtype == np.float64 else 1
+        self._chk(lib.ph_plan3d_create(ctypes.byref(h), nx, ny, nz, *L, code, int(dealias), *filt, gamma))
+        self.h = h
+        n = ctypes.c_size_t()
+        self._chk(lib.ph_plan3d_workspace_bytes(h, ctypes.byref(n)))
+        self.ws = np.zeros(n.value // 8 + 64, dtype=np.float64)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.ph_last_error().decode())
+
+    def deriv(self, axis, f):
+        f = np.ascontiguousarray(f, dtype=self.dtype)
+        out = np.zeros_like(f)
+        nf = f.size // int(np.prod(self.shape))
+        self._chk(self.lib.ph_deriv3d(self.h, axis, nf, P(f), P(out), P(self.ws), None))
+        return out
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        self._chk(self.lib.ph_rhs3d(self.h, P(U), P(out), P(self.ws), None))
+        return out
+
+    def step(self, U, dt):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        self._chk(self.lib.ph_step3d(self.h, P(U), P(out), dt, None, P(self.ws), None))
+        return out
+
+
+class Emu2D:
+    def __init__(self, lib, nx, ny, batch=1, L=(1.0, 1.0), dtype=np.float64, dealias=True,
+                 filt=(1, 16, 36.0), gamma=1.4):
+        self.lib, self.dtype = lib, dtype
+        h = ctypes.c_void_p()
+        code = 0 if dtype == np.float64 else 1
+        self._chk(lib.ph_plan2d_create(ctypes.byref(h), nx, ny, *L, code, int(dealias), *filt, gamma, batch))
+        self.h = h
+        n = ctypes.c_size_t()
+        self._chk(lib.ph_plan2d_workspace_bytes(h, ctypes.byref(n)))
+        self.ws = np.zeros(n.value // 8 + 64, dtype=np.float64)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.ph_last_error().decode())
+
+    def step(self, U, dt, filt=True):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        self._chk(self.lib.ph_step2d(self.h, nb, P(U), P(out), dt, None, None, int(filt), P(self.ws), None))
+        return out
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        self._chk(self.lib.ph_rhs2d(self.h, nb, P(U), P(out), P(self.ws), None))
+        return out

@@ -1,0 +1,179 @@
+"""3D hip backend vs the reference's golden vectors and the oracle.
+
+Tolerance: FP64 results must match to relative L2 <= 1e-11 per conserved
+field (absolute, scaled by the state norm, for a field whose norm is ~0);
+single operators to 1e-13.  FP32 bulk: <= 1e-5 against FP64."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_l2, field_errors, scaled_err, group_errors
+
+pytestmark = pytest.mark.gpu
+FILT = {"enabled": True, "p": 16, "alpha": 36.0}
+
+
+def _setup(nx, ny, nz, precision="double", fp=FILT, L=(2 * np.pi,) * 3):
+    from phrike_amd.grid import Grid3D
+    from phrike_amd.equations import EulerEquations3D
+    from phrike_amd.solver import SpectralSolver3D
+    g = Grid3D(nx, ny, nz, *L, dealias=True, filter_params=dict(fp), backend="hip",
+               precision=precision)
+    e = EulerEquations3D(1.4)
+    return g, e, SpectralSolver3D(g, e, scheme="rk4", cfl=0.4)
+
+
+def _t(a, dev, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dtype or torch.float64)
+
+
+@pytest.mark.parametrize("tag,shape", [("cube16", (16, 16, 16)), ("box16x8x32", (16, 8, 32))])
+def test_operators_vs_reference(dev, tag, shape):
+    d = golden("rhs3d.npz")
+    g, e, s = _setup(*shape)
+    U = _t(d[f"{tag}_U"], dev)
+    print(f"case {tag}", flush=True)
+    _v = scaled_err(g.dx1(U[0]).cpu(), d[f"{tag}_dx_rho"], np.linalg.norm(d[f"{tag}_U"][0]))
+    print("check", 'rel_l2(g.dx1(U[0]).cpu(), d[f"{tag}_dx_rho"])'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+    _v = scaled_err(g.dy1(U[0]).cpu(), d[f"{tag}_dy_rho"], np.linalg.norm(d[f"{tag}_U"][0]))
+    print("check", 'rel_l2(g.dy1(U[0]).cpu(), d[f"{tag}_dy_rho"])'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+    _v = scaled_err(g.dz1(U[0]).cpu(), d[f"{tag}_dz_rho"], np.linalg.norm(d[f"{tag}_U"][0]))
+    print("check", 'rel_l2(g.dz1(U[0]).cpu(), d[f"{tag}_dz_rho"])'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+    _v = rel_l2(g.apply_spectral_filter(U).cpu(), d[f"{tag}_filter"])
+    print("check", 'rel_l2(g.apply_spectral_filter(U).cpu(), d[f"{tag}_filter"])'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+    _v = rel_l2(s.rhs(U).cpu(), d[f"{tag}_rhs"])
+    print("check", 'rel_l2(s.rhs(U).cpu(), d[f"{tag}_rhs"])'[:60], float(_v), flush=True)
+    assert _v < 1e-11
+    assert abs(s.max_wave_speed(U) - float(d[f"{tag}_maxc"])) <= 1e-14 * float(d[f"{tag}_maxc"])
+    dt = s.compute_dt(U)
+    assert abs(dt / float(d[f"{tag}_dt"]) - 1) < 1e-14
+    _v = field_errors(s.step(U, float(d[f"{tag}_dt"])).cpu(), d[f"{tag}_step"]).max()
+    print("check", 'field_errors(s.step(U, float(d[f"{tag}_dt"])).cpu(), d[f"{tag}_step"]).max()'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+
+
+def test_filter_disabled_step(dev):
+    d = golden("rhs3d.npz")
+    g, e, s = _setup(16, 16, 16, fp={"enabled": False})
+    U = _t(d["cube16_U"], dev)
+    _v = field_errors(s.step(U, 1e-3).cpu(), d["nofilter_step"]).max()
+    print("check", 'field_errors(s.step(U, 1e-3).cpu(), d["nofilter_step"]).max()'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_tgv_steps_vs_reference(dev, n):
+    d = golden(f"tgv3d_{n}.npz")
+    g, e, s = _setup(n, n, n)
+    U = _t(d["U0"], dev)
+    dt = float(d["dt"])
+    assert abs(s.compute_dt(U) / dt - 1) < 1e-14
+    for k in range(1, 11):
+        U = s.step(U, dt)
+        if k in (1, 10):
+            _v = field_errors(U.cpu(), d[f"U_{k}"]).max()
+            print("check", 'field_errors(U.cpu(), d[f"U_{k}"]).max()'[:60], float(_v), flush=True)
+            assert _v < 1e-11, k
+
+
+def test_tgv64_200_steps_vs_reference(dev):
+    """200 RK4 steps at 64^3 against the reference's own run (stride-4
+    subsample and conserved sums)."""
+    d = golden("tgv3d_64_200.npz")
+    from phrike_amd.initial_conditions import taylor_green_3d
+    g, e, s = _setup(64, 64, 64)
+    U = _t(taylor_green_3d(64, 64, 64), dev)
+    dt = float(d["dt"])
+    assert abs(s.compute_dt(U) / dt - 1) < 1e-14
+    import torch
+    out = torch.empty_like(U)
+    for _ in range(200):
+        s.step_into(U, out, dt)
+        U, out = out, U
+    errs = field_errors(U[:, ::4, ::4, ::4].cpu(), d["sub4"])
+    print("64^3 x 200 per-field rel L2:", errs, flush=True)
+    assert errs.max() <= 1e-11
+    sums = s.conserved_sums(U)
+    np.testing.assert_allclose(sums[[0, 4]], d["conserved"][[0, 4]], rtol=1e-12)
+
+
+def test_tgv256_vs_reference(dev):
+    """256^3 against the reference's own 256^3 run: stride-8 subsample and
+    the Fourier modes |m| <= 8 after 1, 3 and (when recorded) 200 steps."""
+    d = golden("tgv3d_256_200.npz")
+    from phrike_amd.initial_conditions import taylor_green_3d
+    import torch
+    g, e, s = _setup(256, 256, 256)
+    U = _t(taylor_green_3d(256, 256, 256), dev)
+    dt = float(d["dt"])
+    assert abs(s.compute_dt(U) / dt - 1) < 1e-14
+    out = torch.empty_like(U)
+    checks = [k for k in (1, 3, 200) if f"sub8_{k}" in d.files]
+    for k in range(1, max(checks) + 1):
+        s.step_into(U, out, dt)
+        U, out = out, U
+        if k in checks:
+            sub = U[:, ::8, ::8, ::8].cpu().numpy()
+            errs = field_errors(sub, d[f"sub8_{k}"])
+            low = np.fft.rfftn(U.cpu().numpy(), axes=(-3, -2, -1))
+            idx = np.r_[0:9, -8:0]
+            low = low[:, idx][:, :, idx][:, :, :, :9]
+            lerr = field_errors(low.view(np.float64), d[f"low8_{k}"].view(np.float64))
+            print(f"256^3 step {k}: sub8 {errs.max():.2e} lowmodes {lerr.max():.2e}", flush=True)
+            assert errs.max() <= 1e-11 and lerr.max() <= 1e-11
+
+
+def test_nonuniform_box_vs_oracle(dev):
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import smooth_random_state_3d
+    nx, ny, nz = 64, 32, 128
+    U0 = smooth_random_state_3d(nx, ny, nz, seed=3, kmax=6)
+    og = sn.Grid3DOracle(nx, ny, nz, 2 * np.pi, 2 * np.pi, 2 * np.pi, filter_params=dict(FILT))
+    os_ = sn.Solver3DOracle(og, sn.Euler3DOracle(1.4), cfl=0.4)
+    dt = os_.compute_dt(U0)
+    Uo = os_.step(os_.step(U0, dt), dt)
+    g, e, s = _setup(nx, ny, nz)
+    U = s.step(s.step(_t(U0, dev), dt), dt)
+    _v = field_errors(U.cpu(), Uo).max()
+    print("check", 'field_errors(U.cpu(), Uo).max()'[:60], float(_v), flush=True)
+    assert _v < 1e-12
+
+
+def test_conserved_sums_vs_oracle(dev):
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import smooth_random_state_3d
+    U0 = smooth_random_state_3d(32, 16, 64, seed=4)
+    g, e, s = _setup(32, 16, 64)
+    got = s.conserved_sums(_t(U0, dev))
+    ref = sn.Euler3DOracle(1.4).conserved_quantities(U0)
+    want = [ref["mass"], ref["momentum_x"], ref["momentum_y"], ref["momentum_z"], ref["energy"]]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * abs(want[0]))
+
+
+def test_fp32_bulk_vs_fp64(dev):
+    import torch
+    d = golden("tgv3d_32.npz")
+    g, e, s = _setup(32, 32, 32, precision="single")
+    U = _t(d["U0"], dev, torch.float32)
+    dt = float(d["dt"])
+    for _ in range(10):
+        U = s.step(U, dt)
+    errs = group_errors(U.double().cpu(), d["U_10"], [(0,), (1, 2, 3), (4,)])
+    print("fp32 32^3 x 10 per-field rel L2:", field_errors(U.double().cpu(), d["U_10"]),
+          "group-normalised:", errs, flush=True)
+    assert errs.max() < 1e-5
+
+
+def test_step_rejects_aliasing_and_bad_shapes(dev):
+    import torch
+    from phrike_amd._lib import HipBackendError
+    g, e, s = _setup(16, 16, 16)
+    U = torch.ones((5, 16, 16, 16), dtype=torch.float64, device=dev)
+    with pytest.raises(HipBackendError):
+        s.step_into(U, U, 1e-3)
+    with pytest.raises(ValueError):
+        s.step(torch.ones((4, 16, 16, 16), dtype=torch.float64, device=dev), 1e-3)
