@@ -40,9 +40,22 @@ def byte_model(n: int):
         "yinv": 5 * b + 5 * a, "xinv_stage1": 5 * a + 15, "xinv_stage23": 5 * a + 20,
         "xinv_stage4": 5 * a + 15, "xfwd_copy": 5 + 5 * a, "yfwd_copy": 5 * a + 5 * b,
         "z_copy": 10 * b, "xinv_out": 5 * a + 5,
+        # fused stage boundaries: D in, U in, accumulator, next-stage spectra out
+        "xmid1": 5 * a + 5 + 5 + 15 * a, "xmid23": 5 * a + 5 + 10 + 15 * a,
+        "xmid_last": 5 * a + 5 + 5 + 5 + 5 * a,
     }
+    # per-step model of the unfused pencil pipeline (stage states written and
+    # re-read), the official figure for hbm_frac_step
     C = 4 * (5 + 40 * a + 30 * b) + 70 + 10 + 20 * a + 20 * b
     return per, C
+
+
+def fused_step_units(n: int) -> float:
+    """Bytes per step (units of s*N^3) that the fused pipeline moves."""
+    per, _ = byte_model(n)
+    return (per["xfwd_flux"] + 4 * (per["yfwd_div"] + per["z_div"] + per["yinv"])
+            + per["xmid1"] + 2 * per["xmid23"] + per["xmid_last"]
+            + per["yfwd_copy"] + per["z_copy"] + per["yinv"] + per["xinv_out"])
 
 
 def cpu_baseline(n: int, steps: int):
@@ -69,6 +82,60 @@ def cpu_baseline(n: int, steps: int):
                       f"{cores} FFT workers, {el:.1f} s"}
 
 
+def precision_study(n: int, steps: int, dev):
+    """Decaying turbulence n^3 (seeded random-phase solenoidal velocity,
+    E(k) ~ k^4 exp(-2(k/4)^2), rms Mach 0.3, rho = 1 + 1% noise, uniform p):
+    the same initial state stepped `steps` times in FP64 and in FP32 (FP64
+    tables, sums and time), dt frozen from CFL 0.4 at t = 0.  Reports per-field
+    relative L2 of FP32 against FP64, conservation drifts, and the speedup."""
+    import torch
+    from phrike_amd.equations import EulerEquations3D
+    from phrike_amd.grid import Grid3D
+    from phrike_amd.initial_conditions import decaying_turbulence_3d
+    from phrike_amd.solver import SpectralSolver3D
+
+    U0 = torch.as_tensor(decaying_turbulence_3d(n, seed=1234, density_seed=4321), device=dev)
+    res, finals, sums = {}, {}, {}
+    dt = None
+    for prec in ("double", "single"):
+        g = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                   filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip", precision=prec)
+        s = SpectralSolver3D(g, EulerEquations3D(1.4), scheme="rk4", cfl=0.4)
+        U = U0.to(g.dtype).contiguous()
+        if dt is None:
+            dt = s.compute_dt(U)
+        s0 = s.conserved_sums(U)
+        out = torch.empty_like(U)
+        s.step_into(U, out, dt)  # warm-up step, not timed, not counted
+        U = U0.to(g.dtype).contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step_into(U, out, dt)
+            U, out = out, U
+        torch.cuda.synchronize()
+        res[prec] = (time.perf_counter() - t0) / steps * 1e3
+        s1 = s.conserved_sums(U)
+        sums[prec] = (s0, s1)
+        finals[prec] = U.double()
+        del g, s, out
+    a, b = finals["single"], finals["double"]
+    per = [float(torch.linalg.vector_norm(a[f] - b[f]) / torch.linalg.vector_norm(b[f])) for f in range(5)]
+    mom = max(float(torch.linalg.vector_norm(b[f])) for f in (1, 2, 3))
+    grp = [per[0], *[float(torch.linalg.vector_norm(a[f] - b[f])) / mom for f in (1, 2, 3)], per[4]]
+    s0, s1 = sums["double"]
+    f0, f1 = sums["single"]
+    return {"workload": f"decaying turbulence {n}^3, rms Mach 0.3, {steps} RK4 steps, dt frozen",
+            "ms_per_step_f64": round(res["double"], 3), "ms_per_step_f32": round(res["single"], 3),
+            "speedup_f32": round(res["double"] / res["single"], 3),
+            "rel_l2_f32_vs_f64": [float(f"{e:.3e}") for e in per],
+            "rel_l2_f32_vs_f64_momentum_scale": [float(f"{e:.3e}") for e in grp],
+            "mass_drift_f64": float(f"{(s1[0] - s0[0]) / s0[0]:.3e}"),
+            "energy_drift_f64": float(f"{(s1[4] - s0[4]) / s0[4]:.3e}"),
+            "mass_drift_f32": float(f"{(f1[0] - f0[0]) / f0[0]:.3e}"),
+            "energy_drift_f32": float(f"{(f1[4] - f0[4]) / f0[4]:.3e}")}
+
+
 def load_traffic(kernel: str, n: int, prec: str):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     try:
@@ -89,6 +156,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the FP32-vs-FP64 turbulence study")
+    ap.add_argument("--precision-size", type=int, default=256)
+    ap.add_argument("--precision-steps", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the 3D pseudo-spectral step is not sharded "
@@ -96,6 +169,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": a.numel() * 8,
+                          "bytes_written_per_launch": b.numel() * 8, "launches": 5}))
+        return
     from phrike_amd import _lib
     from phrike_amd.equations import EulerEquations3D
     from phrike_amd.grid import Grid3D
@@ -144,6 +227,8 @@ def main():
     kernels = {}
     dominant = None
     for tag, (cnt, ms) in kt.items():
+        if tag not in per:
+            continue
         avg_ms = ms / cnt
         alg = per.get(tag, 0.0) * es * n ** 3
         kernels[tag] = {"launches": cnt, "avg_ms": round(avg_ms, 4), "total_ms": round(ms, 3),
@@ -159,6 +244,14 @@ def main():
                 "frac": round(ach / HBM_PEAK, 4), "traffic": load_traffic(dominant, n, args.precision),
                 "kernel": dominant, "alg_bytes_per_launch": int(alg)}
     sums = s.conserved_sums(U)
+    del out
+    extras = {}
+    if not args.no_extras:
+        print(f"[bench] precision study: turbulence {args.precision_size}^3 x {args.precision_steps}",
+              file=sys.stderr, flush=True)
+        del U, s, g
+        torch.cuda.empty_cache()
+        extras["fp32_vs_fp64"] = precision_study(args.precision_size, args.precision_steps, dev)
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu baseline: oracle TGV {args.cpu_size}^3 x {args.cpu_steps}", file=sys.stderr,
@@ -173,9 +266,13 @@ def main():
                                f"(2/3 dealias, exp filter alpha=36 p=16), dt frozen from CFL 0.4",
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "hbm_frac_step": round(step_bytes / (ms_per_step / 1e3) / HBM_PEAK, 4),
-        "step_model_bytes": int(step_bytes), "wall_s": round(wall, 3),
+        "step_model_bytes": int(step_bytes),
+        "step_fused_bytes": int(fused_step_units(n) * es * n ** 3),
+        "hbm_frac_step_fused": round(fused_step_units(n) * es * n ** 3 / (ms_per_step / 1e3) / HBM_PEAK, 4),
+        "wall_s": round(wall, 3),
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
         "mass_drift": float(sums[0] / n ** 3 - 1.0),
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

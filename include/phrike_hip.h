@@ -147,6 +147,18 @@ int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max
 int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out4_dev, void* ws,
                         void* stream);
 
+/* Per-run time bookkeeping for device-resident run loops (ensembles): for
+ * every run with active_dev[run] != 0,
+ *   ph_runs_clip:    dt[run] = min(dt[run], t_end[run] - t[run])
+ *   ph_runs_advance: t[run] += dt[run]; steps[run] += 1;
+ *                    active[run] = t[run] < t_end[run] - 1e-12
+ * Replaces the host loop arithmetic of SpectralSolver2D.run / SpectralSolver3D.run
+ * (phrike/solver.py:764-779, 1475-1484).  All FP64 / int32 device arrays. */
+int ph_runs_clip(int nbatch, const double* t_dev, const double* t_end_dev, double* dt_dev,
+                 const int* active_dev, void* stream);
+int ph_runs_advance(int nbatch, double* t_dev, const double* t_end_dev, const double* dt_dev,
+                    int* active_dev, int* steps_dev, void* stream);
+
 /* ------------------------------------------------------------------------
  * Dense LGL collocation derivative on matrix cores:
  *   out[b, i, x] = sum_j D[i, j] F[b, j, x]      (b < nbatch)

@@ -67,7 +67,7 @@ class LegendreLobattoBasis1D:
         self._w_ref = w[order]
         self._x = (1.0 - self._y) * (self.Lx / 2.0)
         self._w_x = self._w_ref * (self.Lx / 2.0)
-        self._D_x = (-2.0 / self.Lx) * _vandermonde_derivative(self._y)
+        self._D_x = np.ascontiguousarray((-2.0 / self.Lx) * _vandermonde_derivative(self._y))
         self._V = L.legvander(self._y, self.N - 1)
         self._proj_scale = (2.0 * np.arange(self.N) + 1.0) / 2.0
         self.torch_device = torch_device
@@ -94,9 +94,11 @@ class LegendreLobattoBasis1D:
 
         if key not in self._dev:
             dev = torch.device(self.torch_device)
-            hi, lo = split_hi_lo(matrix)
-            self._dev[key] = (torch.tensor(matrix, dtype=torch.float64, device=dev),
-                              torch.tensor(hi, device=dev), torch.tensor(lo, device=dev))
+            m = np.ascontiguousarray(matrix, dtype=np.float64)  # row-major for the kernel
+            hi, lo = split_hi_lo(m)
+            self._dev[key] = (torch.from_numpy(m).to(dev).contiguous(),
+                              torch.from_numpy(np.ascontiguousarray(hi)).to(dev).contiguous(),
+                              torch.from_numpy(np.ascontiguousarray(lo)).to(dev).contiguous())
         return self._dev[key]
 
     def apply(self, F, mode: str = "f64", matrix: str = "D", add=None, out=None):

@@ -273,6 +273,26 @@ int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out
   });
 }
 
+int ph_runs_clip(int nbatch, const double* t_dev, const double* t_end_dev, double* dt_dev,
+                 const int* active_dev, void* stream) {
+  return guard([&] {
+    need(nbatch >= 1 && t_dev && t_end_dev && dt_dev && active_dev, "bad arguments");
+    hipLaunchKernelGGL(ph::k_runs_clip, dim3((nbatch + ph::NT - 1) / ph::NT), dim3(ph::NT), 0, S(stream),
+                       nbatch, t_dev, t_end_dev, dt_dev, active_dev);
+    ph::check_launch("k_runs_clip");
+  });
+}
+
+int ph_runs_advance(int nbatch, double* t_dev, const double* t_end_dev, const double* dt_dev,
+                    int* active_dev, int* steps_dev, void* stream) {
+  return guard([&] {
+    need(nbatch >= 1 && t_dev && t_end_dev && dt_dev && active_dev && steps_dev, "bad arguments");
+    hipLaunchKernelGGL(ph::k_runs_advance, dim3((nbatch + ph::NT - 1) / ph::NT), dim3(ph::NT), 0, S(stream),
+                       nbatch, t_dev, t_end_dev, dt_dev, active_dev, steps_dev);
+    ph::check_launch("k_runs_advance");
+  });
+}
+
 int ph_gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
                  const void* F, const void* add, void* out, void* stream) {
   return guard([&] { ph::gll_apply(mode, ny, nx, nbatch, D, Dhi, Dlo, F, add, out, S(stream)); });

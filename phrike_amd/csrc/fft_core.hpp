@@ -22,6 +22,14 @@
 
 namespace ph {
 
+// Barrier for LDS hand-offs only: waits for this wave's LDS traffic but not
+// for its outstanding global loads, so prefetches stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cx {
   T x, y;
@@ -154,7 +162,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
     const int b = tid + q * NT;
     if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
   }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
@@ -166,7 +174,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
       for (int r = 0; r < R; ++r) buf[(d + r * NS) * L + l] = v[q][r];
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 template <typename T, int N, int L, int DIR, int NT, int P, int PEND>
@@ -202,7 +210,7 @@ __device__ __forceinline__ void fft_regs(cx<T>* buf, const cx<T>* tw, int tid,
     const int b = tid + q * NT;
     if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 // As fft_regs, but the first pass takes its inputs from ld(line, position)
@@ -250,7 +258,7 @@ __device__ __forceinline__ void fft_regs_g(cx<T>* buf, const cx<T>* tw, int tid,
         for (int r = 0; r < R0; ++r) buf[(j * R0 + r) * L + l] = w[q][r];
       }
     }
-    __syncthreads();
+    lds_barrier();
     constexpr int LP = NP - 1;
     constexpr int R = Plan<N>::radix(LP);
     constexpr int NS = PassNs<N, LP>::value;
@@ -262,7 +270,82 @@ __device__ __forceinline__ void fft_regs_g(cx<T>* buf, const cx<T>* tw, int tid,
       const int b = tid + q * NT;
       if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
     }
-    __syncthreads();
+    lds_barrier();
+  }
+}
+
+// Pass-0 inputs of a transform: w[q][r] = ld(line, j + r*N/R0) for the
+// butterflies b = tid + q*NT of pass 0.
+template <int N, int L, int NT>
+struct P0Shape {
+  static constexpr int R = Plan<N>::radix(0);
+  static constexpr int NB = L * (N / R);
+  static constexpr int PER = (NB + NT - 1) / NT;
+};
+
+template <typename T, int N, int L, int NT, class Ld>
+__device__ __forceinline__ void load_pass0(int tid, Ld ld,
+                                           cx<T> (&w)[P0Shape<N, L, NT>::PER][P0Shape<N, L, NT>::R]) {
+  using P0 = P0Shape<N, L, NT>;
+#pragma unroll
+  for (int q = 0; q < P0::PER; ++q) {
+    const int b = tid + q * NT;
+    if (P0::NB % NT == 0 || b < P0::NB) {
+      const int l = b % L, j = b / L;
+#pragma unroll
+      for (int r = 0; r < P0::R; ++r) w[q][r] = ld(l, j + r * (N / P0::R));
+    }
+  }
+}
+
+// Transform whose pass-0 inputs were loaded into w beforehand; as soon as w
+// has been consumed, the pass-0 inputs of the next transform are loaded into
+// it (when `more`), so those global loads overlap this transform's LDS
+// passes.  Output as fft_regs (last pass in registers, ends with a barrier).
+template <typename T, int N, int L, int DIR, int NT, class Ld>
+__device__ __forceinline__ void fft_regs_pipe(cx<T>* buf, const cx<T>* tw, int tid,
+                                              cx<T> (&v)[LastRegs<N, L, NT>::PER][LastRegs<N, L, NT>::R],
+                                              cx<T> (&w)[P0Shape<N, L, NT>::PER][P0Shape<N, L, NT>::R],
+                                              bool more, Ld next) {
+  using P0 = P0Shape<N, L, NT>;
+  constexpr int NP = Plan<N>::npass;
+  if constexpr (NP == 1) {
+    static_assert(P0::PER == LastRegs<N, L, NT>::PER, "single pass layout");
+#pragma unroll
+    for (int q = 0; q < P0::PER; ++q) {
+      const int b = tid + q * NT;
+      if (P0::NB % NT == 0 || b < P0::NB) {
+#pragma unroll
+        for (int r = 0; r < P0::R; ++r) v[q][r] = w[q][r];
+        dftR<P0::R, DIR>(v[q]);
+      }
+    }
+    if (more) load_pass0<T, N, L, NT>(tid, next, w);
+  } else {
+#pragma unroll
+    for (int q = 0; q < P0::PER; ++q) {
+      const int b = tid + q * NT;
+      if (P0::NB % NT == 0 || b < P0::NB) {
+        const int l = b % L, j = b / L;
+        dftR<P0::R, DIR>(w[q]);
+#pragma unroll
+        for (int r = 0; r < P0::R; ++r) buf[(j * P0::R + r) * L + l] = w[q][r];
+      }
+    }
+    lds_barrier();
+    if (more) load_pass0<T, N, L, NT>(tid, next, w);
+    constexpr int LP = NP - 1;
+    constexpr int R = Plan<N>::radix(LP);
+    constexpr int NS = PassNs<N, LP>::value;
+    constexpr int NB = L * (N / R);
+    constexpr int PER = LastRegs<N, L, NT>::PER;
+    passes_lds_range<T, N, L, DIR, NT, 1, LP>(buf, tw, tid);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int b = tid + q * NT;
+      if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
+    }
+    lds_barrier();
   }
 }
 

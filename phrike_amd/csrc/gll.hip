@@ -6,9 +6,10 @@
 //
 // One wave64 owns a 16 x 16 output tile and walks K = ny in steps of 4 with
 // v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (split FP32).
-// Operand layout of both instructions (wave64): lane l holds A[l%16][l/16],
-// B[l/16][l%16], and the accumulator register v of lane l is
-// D[4*(l/16)+v][l%16].
+// Operand layout (wave64): lane l holds A[l%16][l/16] and B[l/16][l%16] for
+// both instructions; accumulator register v of lane l is D[4*(l/16)+v][l%16]
+// for the FP32 instruction and D[(l/16)+4*v][l%16] for the FP64 one
+// (checked on the device with single-entry matrices).
 //
 // Split precision: D = Dhi + Dlo with Dhi = fp32(D), Dlo = fp32(D - Dhi)
 // (built on the host from the FP64 table).  For FP32 data the product is
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_f64(const double* __rest
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const long long o = bo + (long long)(i0 + 4 * q + v) * nx + x0 + r;
+    const long long o = bo + (long long)(i0 + q + 4 * v) * nx + x0 + r;
     double val = acc[v];
     if constexpr (EPI == 1) val = -(add[o] + val);
     out[o] = val;

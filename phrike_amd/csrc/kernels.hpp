@@ -24,6 +24,8 @@
 // All spectral buffers hold complex cx<T>; row pitch mxp >= mx keeps rows
 // aligned.  Every kernel uses 256 threads (four wave64s).
 #pragma once
+#include <type_traits>
+
 #include "fft_core.hpp"
 
 namespace ph {
@@ -75,34 +77,16 @@ template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 
-template <typename T, int NX, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a) {
-  constexpr int NT = XThreads<T, NX>::value;
+// Fluxes (or copies) of the conserved values q held in registers -> forward
+// x transform of two rows at once -> kept modes to a.out.  Shared by k_xfwd
+// and the fused k_xmid.
+template <typename T, int NX, int MODE, int NT, int PPT>
+__device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, cx<T>* tw, cx<T>* buf,
+                                               int tid, int slab, int y0,
+                                               const T (&q)[PPT][XFwdShape<MODE>::NC]) {
   using S = XFwdShape<MODE>;
   constexpr int LX = S::LX, NC = S::NC;
-  constexpr int PPT = (2 * NX + NT - 1) / NT;
-  __shared__ cx<T> tw[NX];
-  __shared__ cx<T> buf[LX * NX];
   T* bufr = reinterpret_cast<T*>(buf);
-  const int tid = threadIdx.x;
-  const int slab = blockIdx.y;
-  const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
-
-  const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
-  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
-  T q[PPT][NC];
-#pragma unroll
-  for (int pi = 0; pi < PPT; ++pi) {
-    const int p = tid + pi * NT;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) q[pi][c] = T(0);
-    if (p < 2 * NX) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        if (c < ncomp) q[pi][c] = src[(long long)c * a.in_fs + p];
-    }
-  }
   // primitive recovery (phrike/equations.py:104-117, 150-164)
   T pr[PPT][6];
   if constexpr (MODE != XF_COPY) {
@@ -155,7 +139,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
     fft_lds<T, NX, LX, -1, NT>(buf, tw, tid);
     // split the two real spectra, keep mx modes, scale, write two rows
     for (int i = tid; i < LX * a.mxp; i += NT) {
@@ -181,8 +165,38 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a
       o[0] = A;
       o[a.mxp] = B;
     }
-    __syncthreads();
+    lds_barrier();
   }
+}
+
+template <typename T, int NX, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a) {
+  constexpr int NT = XThreads<T, NX>::value;
+  using S = XFwdShape<MODE>;
+  constexpr int LX = S::LX, NC = S::NC;
+  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  __shared__ cx<T> tw[NX];
+  __shared__ cx<T> buf[LX * NX];
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  const int y0 = 2 * blockIdx.x;
+  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+
+  const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
+  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
+  T q[PPT][NC];
+#pragma unroll
+  for (int pi = 0; pi < PPT; ++pi) {
+    const int p = tid + pi * NT;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[pi][c] = T(0);
+    if (p < 2 * NX) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        if (c < ncomp) q[pi][c] = src[(long long)c * a.in_fs + p];
+    }
+  }
+  xfwd_from_regs<T, NX, MODE, NT, PPT>(a, ncomp, tw, buf, tid, slab, y0, q);
 }
 
 // ----------------------------------------------------------------------------
@@ -231,11 +245,17 @@ struct ColTile {
 #define PH_COL_WAVES_PER_EU 2
 #endif
 
+// Per block, the global-source transforms run in a fixed order (field
+// sequence below); the pass-0 inputs of transform k+1 are loaded while
+// transform k runs its LDS passes.  For the two-input modes the first
+// spectrum stays in registers (Y) while the second is transformed, so every
+// output is written once.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
   constexpr int TM = ColTile<T, N>::TM;
   constexpr int NT = ColTile<T, N>::NT;
   using LR = LastRegs<N, TM, NT>;
+  using P0 = P0Shape<N, TM, NT>;
   constexpr int PER = LR::PER, R = LR::R;
   __shared__ cx<T> tw[N];
   __shared__ cx<T> buf[N * TM];
@@ -245,115 +265,123 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
   const cx<T>* in = a.in + (long long)blockIdx.y * a.in_bs;
   cx<T>* out = a.out + (long long)blockIdx.y * a.out_bs;
   const Keep kp = a.keep;
-  cx<T> X[PER][R], Z[PER][R];
-
-  const long long ls = a.in_ls;
+  const long long ls = a.in_ls, ols = a.out_ls;
   const int ncol = a.ncol;
-  // element (line l, position n) of field f's tile, zero outside the grid
-  auto full = [&](int f) {
-    const cx<T>* src = in + (long long)f * a.in_fs + c0;
-    return [=](int l, int n) { return (c0 + l < ncol) ? src[(long long)n * ls + l] : mk(T(0), T(0)); };
+  const int nc = a.nc;
+  cx<T> X[PER][R];
+  cx<T> Y[(MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) ? PER : 1][R];  // first spectrum of a pair
+  cx<T> W[P0::PER][P0::R];
+  auto copy_regs = [&](auto& dst, const auto& src) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[q][r] = src[q][r];
   };
-  __syncthreads();  // twiddles
-#pragma unroll 1
-  for (int c = 0; c < a.nc; ++c) {
-    if constexpr (MODE == CM_INV) {
-      const cx<T>* src = in + (long long)c * a.in_fs + c0;
-      fft_regs_g<T, N, TM, +1, NT>(buf, tw, tid, X, [=](int l, int n) {
+
+  // field read by the k-th global-source transform of this block
+  auto field_of = [&](int k) {
+    if constexpr (MODE == CM_FWD_DIV3) return (k % 3) * 5 + k / 3;
+    else if constexpr (MODE == CM_FWDINV_DIV) return (k % 2) * nc + k / 2;
+    else return k;
+  };
+  constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : (MODE == CM_FWDINV_DIV ? 2 : 1);
+  const int nsrc = PERC * nc;
+  // element (line l, position n) of field f's tile, zero outside the grid
+  auto loader = [&](int k) {
+    const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
+    return [=](int l, int n) {
+      if constexpr (MODE == CM_INV)
         return (c0 + l < ncol && kp.kept(n, N)) ? src[(long long)kp.compact(n, N) * ls + l] : mk(T(0), T(0));
+      else
+        return (c0 + l < ncol) ? src[(long long)n * ls + l] : mk(T(0), T(0));
+    };
+  };
+  // visit every (q, r) output slot of this thread: f(q, r, l, n, col_ok)
+  auto each = [&](auto&& f) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (!last_valid<N, TM, NT>(tid, q)) continue;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        int l, n;
+        last_pos<N, TM, NT>(tid, q, r, l, n);
+        f(q, r, l, n, c0 + l < ncol);
+      }
+    }
+  };
+  int k = 0;
+  auto fwd = [&](auto dir) {  // next global-source transform into X
+    constexpr int D = decltype(dir)::value;
+    const bool more = k + 1 < nsrc;
+    fft_regs_pipe<T, N, TM, D, NT>(buf, tw, tid, X, W, more, loader(more ? k + 1 : k));
+    ++k;
+  };
+  using FWD = std::integral_constant<int, -1>;
+  using INV = std::integral_constant<int, +1>;
+  load_pass0<T, N, TM, NT>(tid, loader(0), W);
+  lds_barrier();  // twiddles
+#pragma unroll 1
+  for (int c = 0; c < nc; ++c) {
+    cx<T>* oc = out + (long long)c * a.out_fs + c0;
+    if constexpr (MODE == CM_INV) {
+      fwd(INV{});
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (ok) oc[(long long)n * ols + l] = X[q][r];
+      });
+    } else if constexpr (MODE == CM_FWD_COPY) {
+      fwd(FWD{});
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (ok && kp.kept(n, N)) {
+          const int j = kp.compact(n, N);
+          cx<T> v = cscale(X[q][r], a.sig[j]);
+          if (a.ikflag) v = mul_ik(v, a.kk[j]);
+          oc[(long long)j * ols + l] = v;
+        }
+      });
+    } else if constexpr (MODE == CM_FWD_DIV3) {
+      // G = sigma (X + i k Z) with X = F(Fx^), Z = F(Fy^); then sigma F(Fz^)
+      fwd(FWD{});
+      copy_regs(Y, X);
+      fwd(FWD{});
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (ok && kp.kept(n, N)) {
+          const int j = kp.compact(n, N);
+          oc[(long long)j * ols + l] = cscale(Y[q][r] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+        }
+      });
+      cx<T>* o2 = out + (long long)(5 + c) * a.out_fs + c0;
+      fwd(FWD{});
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (ok && kp.kept(n, N)) {
+          const int j = kp.compact(n, N);
+          o2[(long long)j * ols + l] = cscale(X[q][r], a.sig[j]);
+        }
       });
     } else {
-      fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, X, full(c));
-      if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) {
-        const int second = (MODE == CM_FWD_DIV3) ? 5 + c : a.nc + c;
-        fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, Z, full(second));
+      // CM_FWDINV_DIV / CM_FWDINV_COPY: forward, combine on kept modes, inverse
+      fwd(FWD{});
+      if constexpr (MODE == CM_FWDINV_DIV) {
+        copy_regs(Y, X);
+        fwd(FWD{});
       }
-    }
-    // combine on the forward outputs (all modes but CM_INV)
-    if constexpr (MODE != CM_INV) {
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (!last_valid<N, TM, NT>(tid, q)) continue;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          int l, n;
-          last_pos<N, TM, NT>(tid, q, r, l, n);
-          const bool kept = kp.kept(n, N);
-          cx<T> v = mk(T(0), T(0));
-          if (kept) {
-            const int j = kp.compact(n, N);
-            const T s = a.sig[j];
-            if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) {
-              v = cscale(X[q][r] + mul_ik(Z[q][r], a.kk[j]), s);
-            } else {
-              v = cscale(X[q][r], s);
-              if (a.ikflag) v = mul_ik(v, a.kk[j]);
-            }
-          }
-          X[q][r] = v;
-        }
-      }
-    }
-    if constexpr (MODE == CM_FWD_DIV3 || MODE == CM_FWD_COPY) {
-      cx<T>* o = out + (long long)c * a.out_fs;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (!last_valid<N, TM, NT>(tid, q)) continue;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          int l, n;
-          last_pos<N, TM, NT>(tid, q, r, l, n);
-          const int col = c0 + l;
-          if (col < a.ncol && kp.kept(n, N)) o[(long long)kp.compact(n, N) * a.out_ls + col] = X[q][r];
-        }
-      }
-      if constexpr (MODE == CM_FWD_DIV3) {
-        // sigma_y * Fz^ (second output group)
-        fft_regs_g<T, N, TM, -1, NT>(buf, tw, tid, Z, full(10 + c));
-        cx<T>* o2 = out + (long long)(5 + c) * a.out_fs;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          if (!last_valid<N, TM, NT>(tid, q)) continue;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            int l, n;
-            last_pos<N, TM, NT>(tid, q, r, l, n);
-            const int col = c0 + l;
-            if (col < a.ncol && kp.kept(n, N)) {
-              const int j = kp.compact(n, N);
-              o2[(long long)j * a.out_ls + col] = cscale(Z[q][r], a.sig[j]);
-            }
+      each([&](int q, int r, int l, int n, bool ok) {
+        cx<T> v = mk(T(0), T(0));
+        if (kp.kept(n, N)) {
+          const int j = kp.compact(n, N);
+          if constexpr (MODE == CM_FWDINV_DIV) {
+            v = cscale(Y[q][r] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+          } else {
+            v = cscale(X[q][r], a.sig[j]);
+            if (a.ikflag) v = mul_ik(v, a.kk[j]);
           }
         }
-      }
-    } else {
-      if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_COPY) {
-        // spectral result back to LDS in natural order, then inverse
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          if (!last_valid<N, TM, NT>(tid, q)) continue;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            int l, n;
-            last_pos<N, TM, NT>(tid, q, r, l, n);
-            buf[n * TM + l] = X[q][r];
-          }
-        }
-        __syncthreads();
-        fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
-      }
-      cx<T>* o = out + (long long)c * a.out_fs;
-#pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        if (!last_valid<N, TM, NT>(tid, q)) continue;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          int l, n;
-          last_pos<N, TM, NT>(tid, q, r, l, n);
-          const int col = c0 + l;
-          if (col < a.ncol) o[(long long)n * a.out_ls + col] = X[q][r];
-        }
-      }
+        buf[n * TM + l] = v;
+      });
+      lds_barrier();
+      fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (ok) oc[(long long)n * ols + l] = X[q][r];
+      });
     }
   }
 }
@@ -401,20 +429,12 @@ __device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
   u[3] = pc / (gamma - T(1)) + (T(0.5) * rc) * (ux * ux + uy * uy);
 }
 
-template <typename T, int NX, int NC, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a) {
-  constexpr int NT = XThreads<T, NX>::value;
-  constexpr int PPT = (2 * NX + NT - 1) / NT;
-  __shared__ cx<T> tw[NX];
-  __shared__ cx<T> buf[NC * NX];
-  const int tid = threadIdx.x;
-  const int slab = blockIdx.y;
-  if (a.active != nullptr && a.active[slab] == 0) return;
-  const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
-  const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
+// Rebuild z = row0 + i*row1 from the kept half spectra of both rows of nfl
+// fields and inverse-transform along x; result in buf (natural order).
+template <typename T, int NX, int NC, int NT>
+__device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, cx<T>* tw, cx<T>* buf, int tid,
+                                            int slab, int y0) {
   const cx<T>* src = a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp;
-  // rebuild z = row0 + i*row1 from the kept half spectra of both rows
   for (int i = tid; i < NC * NX; i += NT) {
     const int l = i / NX, n = i - l * NX;
     cx<T> z = mk(T(0), T(0));
@@ -431,12 +451,55 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
     }
     buf[n * NC + l] = z;
   }
-  __syncthreads();
+  lds_barrier();
   fft_lds<T, NX, NC, +1, NT>(buf, tw, tid);
+}
+
+// RK4 stage combine at one point (phrike/solver.py:1220-1230 3D, 502-513 2D)
+// with k_s = -v: stages 1-3 update the accumulator a.out and return the next
+// stage state in u; stage 4 returns U + dt/6 (acc + k4) in u.  The 2D form
+// clamps u (phrike/solver.py:467-488).
+template <typename T, int NC, bool CLAMP>
+__device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC], long long o, double dtd,
+                                         T (&u)[NC]) {
+  const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0);
+  if (a.stage < 4) {
+    const T coef = (a.stage == 3) ? full : half;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const T k = -v[c];
+      const long long idx = c * a.fs + o;
+      const T un = a.un[idx];
+      if (a.stage == 1) a.out[idx] = k;
+      else a.out[idx] = a.out[idx] + T(2) * k;
+      u[c] = un + coef * k;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const long long idx = c * a.fs + o;
+      u[c] = a.un[idx] + sixth * (a.out[idx] + (-v[c]));
+    }
+  }
+  if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
+}
+
+template <typename T, int NX, int NC, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a) {
+  constexpr int NT = XThreads<T, NX>::value;
+  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  __shared__ cx<T> tw[NX];
+  __shared__ cx<T> buf[NC * NX];
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (a.active != nullptr && a.active[slab] == 0) return;
+  const int y0 = 2 * blockIdx.x;
+  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
+  xinv_to_lds<T, NX, NC, NT>(a, nfl, tw, buf, tid, slab, y0);
   const T* bufr = reinterpret_cast<const T*>(buf);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
-  double dtd = a.dt;
-  if (a.dt_dev != nullptr) dtd = a.dt_dev[slab];
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
 #pragma unroll
   for (int pi = 0; pi < PPT; ++pi) {
     const int p = tid + pi * NT;
@@ -455,34 +518,76 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
       for (int c = 0; c < NC; ++c)
         if (c < nfl) a.out[c * a.fs + o] = -v[c];
     } else {
-      // phrike/solver.py:1220-1230 (3D), 502-513 (2D): k_s = -value
-      const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0);
       T u[NC];
-      if (a.stage < 4) {
-        const T coef = (a.stage == 3) ? full : half;
+      rk_point<T, NC, MODE == XI_STAGE2>(a, v, o, dtd, u);
+      T* dst = a.stage < 4 ? a.us : a.out;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const T k = -v[c];
-          const long long idx = c * a.fs + o;
-          const T un = a.un[idx];
-          if (a.stage == 1) a.out[idx] = k;
-          else a.out[idx] = a.out[idx] + T(2) * k;
-          u[c] = un + coef * k;
-        }
-        if constexpr (MODE == XI_STAGE2) clamp2d(u, a.gamma, a.rho_min, a.p_min);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) a.us[c * a.fs + o] = u[c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          const long long idx = c * a.fs + o;
-          u[c] = a.un[idx] + sixth * (a.out[idx] + (-v[c]));
-        }
-        if constexpr (MODE == XI_STAGE2) clamp2d(u, a.gamma, a.rho_min, a.p_min);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
-      }
+      for (int c = 0; c < NC; ++c) dst[c * a.fs + o] = u[c];
     }
+  }
+}
+
+// Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
+// the fluxes of the stage s+1 state and their forward x transform, or (stage
+// 4) the new state written to a.out and its forward x transform for the
+// post-step filter.  The intermediate stage state never goes to HBM.  The
+// spectra are read from and written to the same rows of the same buffer; the
+// block loads its rows completely before it writes them.
+template <typename T>
+struct XMidArgs {
+  XInvArgs<T> inv;
+  XFwdArgs<T> fwd;       // stages 1-3
+  XFwdArgs<T> fwd_last;  // stage 4 (copy mode, 5 or 4 fields)
+};
+
+template <typename T, int NX, int NC, int FMODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m) {
+  constexpr int NT = XThreads<T, NX>::value;
+  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  constexpr int LB = NC > XFwdShape<FMODE>::LX ? NC : XFwdShape<FMODE>::LX;
+  static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
+  __shared__ cx<T> tw[NX];
+  __shared__ cx<T> buf[(LB > 5 ? LB : 5) * NX];
+  const XInvArgs<T>& a = m.inv;
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (a.active != nullptr && a.active[slab] == 0) return;
+  const int y0 = 2 * blockIdx.x;
+  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  xinv_to_lds<T, NX, NC, NT>(a, NC, tw, buf, tid, slab, y0);
+  const T* bufr = reinterpret_cast<const T*>(buf);
+  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
+  T q[PPT][5];
+#pragma unroll
+  for (int pi = 0; pi < PPT; ++pi) {
+    const int p = tid + pi * NT;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[pi][c] = T(0);
+    if (p >= 2 * NX) continue;
+    const int line = p / NX, x = p - line * NX;
+    const long long o = base + p;
+    T v[NC], u[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * NC + c) + line];
+    rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
+    if (a.stage == 4) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) q[pi][c] = u[c];
+  }
+  lds_barrier();  // buf is reused by the forward transform
+  if (a.stage < 4) {
+    T qf[PPT][XFwdShape<FMODE>::NC];
+#pragma unroll
+    for (int pi = 0; pi < PPT; ++pi)
+#pragma unroll
+      for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[pi][c] = q[pi][c];
+    xfwd_from_regs<T, NX, FMODE, NT, PPT>(m.fwd, NC, tw, buf, tid, slab, y0, qf);
+  } else {
+    xfwd_from_regs<T, NX, XF_COPY, NT, PPT>(m.fwd_last, NC, tw, buf, tid, slab, y0, q);
   }
 }
 
@@ -529,10 +634,10 @@ __global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long f
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
     m = dmax_nan(m, wave_speed<T, DIM>(us, fs, i, gamma));
   red[threadIdx.x] = m;
-  __syncthreads();
+  lds_barrier();
   for (int s = NT / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = dmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
@@ -546,10 +651,10 @@ __global__ __launch_bounds__(NT) void k_max_final(const double* part, int nb, do
   double m = 0.0;
   for (int i = threadIdx.x; i < nb; i += NT) m = dmax_nan(m, ps[i]);
   red[threadIdx.x] = m;
-  __syncthreads();
+  lds_barrier();
   for (int s = NT / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = dmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x == 0) {
     if (out) out[blockIdx.x] = red[0];
@@ -591,13 +696,13 @@ __global__ __launch_bounds__(NT) void k_sums_partial(const T* u, long long fs, l
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s) red[s][threadIdx.x] = acc[s];
-  __syncthreads();
+  lds_barrier();
   for (int w = NT / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
 #pragma unroll
       for (int s = 0; s < NS; ++s) red[s][threadIdx.x] += red[s][threadIdx.x + w];
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x < NS)
     part[((long long)blockIdx.y * gridDim.x + blockIdx.x) * NS + threadIdx.x] = red[threadIdx.x][0];
@@ -616,15 +721,40 @@ __global__ __launch_bounds__(NT) void k_sums_final(const double* part, int nb, d
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s) red[s][threadIdx.x] = acc[s];
-  __syncthreads();
+  lds_barrier();
   for (int w = NT / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
 #pragma unroll
       for (int s = 0; s < NS; ++s) red[s][threadIdx.x] += red[s][threadIdx.x + w];
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x < NS) out[(long long)blockIdx.x * NS + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ----------------------------------------------------------------------------
+// per-run time bookkeeping of the run loop (phrike/solver.py:764-797):
+//   dt = min(dt, t_end - t) before a step; t += dt and
+//   active = t < t_end - 1e-12 after it.  FP64 throughout.
+// ----------------------------------------------------------------------------
+static __global__ __launch_bounds__(NT) void k_runs_clip(int nb, const double* t, const double* t_end, double* dt,
+                                                  const int* active) {
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r < nb && active[r]) {
+    const double rem = t_end[r] - t[r];
+    if (rem < dt[r]) dt[r] = rem;
+  }
+}
+
+static __global__ __launch_bounds__(NT) void k_runs_advance(int nb, double* t, const double* t_end,
+                                                     const double* dt, int* active, int* steps) {
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r < nb && active[r]) {
+    const double tn = t[r] + dt[r];
+    t[r] = tn;
+    steps[r] += 1;
+    active[r] = tn < t_end[r] - 1e-12 ? 1 : 0;
+  }
 }
 
 }  // namespace ph

@@ -141,6 +141,12 @@ void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t
   PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xinv<T, N_, NC, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
 }
 
+template <typename T, int NC, int FMODE>
+void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
+  dim3 grid(nrows / 2, nslab);
+  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xmid<T, N_, NC, FMODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
+}
+
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
 // Optional per-kernel timing: when enabled, every pipeline launch is
@@ -236,7 +242,7 @@ struct Plan3D : PlanBase {
     ay = AxisTables(ny, ly, dealias, filt, p, alpha, false);
     az = AxisTables(nz, lz, dealias, filt, p, alpha, false);
     mx = ax.nr;
-    const int align = sizeof(T) == 8 ? 4 : 8;  // 64-byte rows
+    const int align = sizeof(T) == 8 ? 8 : 16;  // 128-byte rows (cache lines)
     mxp = (mx + align - 1) / align * align;
     ncell = (long long)nx * ny * nz;
     const double scale = 1.0 / (double(nx) * double(ny) * double(nz));
@@ -246,12 +252,11 @@ struct Plan3D : PlanBase {
     const size_t cs = sizeof(cx<T>);
     const size_t szA = 15ull * nz * ny * mxp * cs;
     const size_t szB = 10ull * nz * ay.nr * mxp * cs;
-    const size_t szU = 5ull * ncell * sizeof(T);
     auto al = [](size_t v) { return (v + 255) / 256 * 256; };
     offA = 0;
     offB = al(offA + szA);
-    offUs = al(offB + szB);
-    offPart = al(offUs + szU);
+    offUs = offB;  // unused: the fused step keeps stage states in registers
+    offPart = al(offB + szB);
     wsBytes = al(offPart + sizeof(double) * NPART * 5);
   }
   ~Plan3D() override {
@@ -268,6 +273,24 @@ struct Plan3D : PlanBase {
   long long a_fs() const { return (long long)nz * ny * mxp; }
   long long b_fs() const { return (long long)nz * ay.nr * mxp; }
 
+  XFwdArgs<T> xfwd_args(const T* in, int nf, unsigned ikmask, void* ws) const {
+    XFwdArgs<T> a{};
+    a.in = in;
+    a.in_fs = ncell;
+    a.in_ss = (long long)ny * nx;
+    a.out = A(ws);
+    a.out_fs = a_fs();
+    a.out_ss = (long long)ny * mxp;
+    a.nf = nf;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.ikmask = ikmask;
+    a.tw = dx.tw;
+    a.sig = dx.sig;
+    a.kk = dx.k;
+    a.gamma = T(gamma);
+    return a;
+  }
   // forward x pass
   void xfwd(int mode, const T* in, int nf, unsigned ikmask, void* ws, hipStream_t st) const {
     XFwdArgs<T> a{};
@@ -404,24 +427,38 @@ struct Plan3D : PlanBase {
     check_launch("k_xinv");
   }
   // RK4 (phrike/solver.py:1220-1230) then the post-step filter (1233-1234,
-  // 1336).  `out` doubles as the stage accumulator.
+  // 1336).  `out` doubles as the stage accumulator; each stage boundary is
+  // one fused k_xmid launch (x inverse + combine + next fluxes + x forward).
   void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st) const {
-    T* us = Us(ws);
+    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st);
     for (int s = 1; s <= 4; ++s) {
-      divergence(s == 1 ? U : us, ws, st);
-      XInvArgs<T> a = xinv_args(ws);
-      a.nf = 5;
-      a.out = out;
-      a.un = U;
-      a.us = us;
-      a.stage = s;
-      a.dt = dt;
-      a.dt_dev = dt_dev;
-      KScope ks(s == 4 ? "xinv_stage4" : (s == 1 ? "xinv_stage1" : "xinv_stage23"), st);
-      launch_xinv<T, 5, XI_STAGE3>(nx, ny, nz, a, st);
-      check_launch("k_xinv(stage)");
+      yfwd(true, 5, 0, ws, st);
+      zpass(true, 5, 0, ws, st);
+      yinv(5, ws, st);
+      XMidArgs<T> m{};
+      m.inv = xinv_args(ws);
+      m.inv.nf = 5;
+      m.inv.out = out;
+      m.inv.un = U;
+      m.inv.stage = s;
+      m.inv.dt = dt;
+      m.inv.dt_dev = dt_dev;
+      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws);
+      m.fwd_last = xfwd_args(nullptr, 5, 0u, ws);
+      KScope ks(s == 4 ? "xmid_last" : (s == 1 ? "xmid1" : "xmid23"), st);
+      launch_xmid<T, 5, XF_FLUX3>(nx, ny, nz, m, st);
+      check_launch("k_xmid");
     }
-    deriv(-1, 5, out, out, ws, st);
+    // remaining passes of the post-step filter on the new state
+    yfwd(false, 5, 0, ws, st);
+    zpass(false, 5, 0, ws, st);
+    yinv(5, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = 5;
+    a.out = out;
+    KScope ks("xinv_out", st);
+    launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+    check_launch("k_xinv");
   }
   void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
     double* part = Part(ws);
@@ -467,7 +504,7 @@ struct Plan2D : PlanBase {
     ax = AxisTables(nx, lx, dealias, filt, p, alpha, true);
     ay = AxisTables(ny, ly, dealias, filt, p, alpha, false);
     mx = ax.nr;
-    const int align = sizeof(T) == 8 ? 4 : 8;
+    const int align = sizeof(T) == 8 ? 8 : 16;  // 128-byte rows (cache lines)
     mxp = (mx + align - 1) / align * align;
     ncell = (long long)nx * ny;
     dx.upload(ax, 1.0 / (double(nx) * double(ny)));
@@ -489,6 +526,23 @@ struct Plan2D : PlanBase {
 
   void check_nb(int nb) const {
     if (nb < 1 || nb > batch) throw Error(-2, "nbatch must be in [1, plan batch]");
+  }
+  XFwdArgs<T> xfwd_args(unsigned ikmask, void* ws) const {
+    XFwdArgs<T> a{};
+    a.in_fs = ncell;
+    a.in_ss = 4 * ncell;
+    a.out = A(ws);
+    a.out_fs = (long long)ny * mxp;
+    a.out_ss = 8ll * ny * mxp;
+    a.nf = 4;
+    a.mx = mx;
+    a.mxp = mxp;
+    a.ikmask = ikmask;
+    a.tw = dx.tw;
+    a.sig = dx.sig;
+    a.kk = dx.k;
+    a.gamma = T(gamma);
+    return a;
   }
   void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st) const {
     XFwdArgs<T> a{};
@@ -566,26 +620,39 @@ struct Plan2D : PlanBase {
     check_launch("k_xinv(2d rhs)");
   }
   // RK4 with clamp (phrike/solver.py:502-513) and the filter when asked
-  // (cadence decided by the caller, phrike/solver.py:645-647)
+  // (cadence decided by the caller, phrike/solver.py:645-647); stage
+  // boundaries fused as in the 3D step
   void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active,
             bool filter, void* ws, hipStream_t st) const {
-    T* us = Us(ws);
+    xfwd(XF_FLUX2, U, nb, 0xFu, ws, st);
     for (int s = 1; s <= 4; ++s) {
-      xfwd(XF_FLUX2, s == 1 ? U : us, nb, 0xFu, ws, st);
       ypass(true, 0, nb, ws, st);
-      XInvArgs<T> a = xinv_args(ws);
-      a.nf = 4;
-      a.out = out;
-      a.un = U;
-      a.us = us;
-      a.stage = s;
-      a.dt = dt;
-      a.dt_dev = dt_dev;
-      a.active = active;
-      launch_xinv<T, 4, XI_STAGE2>(nx, ny, nb, a, st);
-      check_launch("k_xinv(2d stage)");
+      XMidArgs<T> m{};
+      m.inv = xinv_args(ws);
+      m.inv.nf = 4;
+      m.inv.out = out;
+      m.inv.un = U;
+      m.inv.stage = s;
+      m.inv.dt = dt;
+      m.inv.dt_dev = dt_dev;
+      m.inv.active = active;
+      if (s == 4 && !filter) {
+        launch_xinv<T, 4, XI_STAGE2>(nx, ny, nb, m.inv, st);
+        check_launch("k_xinv(2d stage 4)");
+        return;
+      }
+      m.fwd = xfwd_args(0xFu, ws);
+      m.fwd_last = xfwd_args(0u, ws);
+      launch_xmid<T, 4, XF_FLUX2>(nx, ny, nb, m, st);
+      check_launch("k_xmid(2d)");
     }
-    if (filter) deriv(-1, nb, out, out, active, ws, st);
+    ypass(false, 0, nb, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = 4;
+    a.out = out;
+    a.active = active;
+    launch_xinv<T, 4, XI_OUT>(nx, ny, nb, a, st);
+    check_launch("k_xinv(2d filter)");
   }
   void max_speed(int nb, const T* U, double* out, double* dt_out, double cfl, void* ws,
                  hipStream_t st) const {
