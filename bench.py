@@ -101,13 +101,13 @@ def precision_study(n: int, steps: int, dev):
         g = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
                    filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip", precision=prec)
         s = SpectralSolver3D(g, EulerEquations3D(1.4), scheme="rk4", cfl=0.4)
-        U = U0.to(g.dtype).contiguous()
+        U = U0.to(g.dtype).clone()  # never alias U0: the loop writes both buffers
         if dt is None:
             dt = s.compute_dt(U)
         s0 = s.conserved_sums(U)
         out = torch.empty_like(U)
         s.step_into(U, out, dt)  # warm-up step, not timed, not counted
-        U = U0.to(g.dtype).contiguous()
+        U = U0.to(g.dtype).clone()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -136,6 +136,96 @@ def precision_study(n: int, steps: int, dev):
             "energy_drift_f32": float(f"{(f1[4] - f0[4]) / f0[4]:.3e}")}
 
 
+def hybrid_study(nx: int, nys, steps: int, dev):
+    """2D Kelvin-Helmholtz on the hybrid grid (Fourier x on [0,1), LGL y on
+    [-1/2, 1/2]): shear layers at y = +-1/4 of width 0.05, rho 1 -> 2, p = 2.5,
+    v = 0.01 sin(4 pi x) + 1e-4 N(0,1) (seed 7); `steps` RK4 steps with dt
+    frozen from CFL 0.4 at t = 0, post-step x/y filters.  FP64 with the FP64
+    matrix-core D, then the same with split-precision D (FP32 matrix cores on
+    hi/lo parts of D and of the FP64 data) and with an FP32 state; reports
+    per-field relative L2 against FP64, time per step, and the D.U kernel
+    rate (useful flops 2 Ny^2 * 4 Nx per apply)."""
+    import torch
+    from phrike_amd import _lib
+    from phrike_amd.equations import EulerEquations2D
+    from phrike_amd.hybrid import HybridGrid2D, HybridSolver2D
+    from phrike_amd.initial_conditions import khi_2d_gll
+
+    fp = {"enabled": True, "p": 16, "alpha": 36.0}
+    out = {}
+    for ny in nys:
+        finals, ms, dk = {}, {}, {}
+        dt = None
+        for name, prec, gll in (("f64", "double", "f64"), ("split_f64state", "double", "split"),
+                                ("split_f32state", "single", "split")):
+            g = HybridGrid2D(nx, ny, filter_params=fp, precision=prec, gll=gll, torch_device=str(dev))
+            sol = HybridSolver2D(g, EulerEquations2D(1.4), cfl=0.4)
+            U0 = torch.as_tensor(khi_2d_gll(nx, g.y_nodes, seed=7), device=dev).to(g.dtype)
+            if dt is None:
+                dt = sol.compute_dt(U0)
+            sol.advance(U0, 2, dt=dt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            U = sol.advance(U0, steps, dt=dt)
+            torch.cuda.synchronize()
+            ms[name] = (time.perf_counter() - t0) / steps * 1e3
+            # per-kernel times from a short separate pass (events per launch)
+            _lib.timing_collect()
+            _lib.timing_enable(True)
+            sol.advance(U0, 20, dt=dt)
+            torch.cuda.synchronize()
+            _lib.timing_enable(False)
+            kt = _lib.timing_collect()
+            if "h_gll_d" in kt:
+                cnt, tms = kt["h_gll_d"]
+                flops = 2.0 * ny * ny * 4 * nx
+                dk[name] = {"avg_us": round(tms / cnt * 1e3, 2),
+                            "useful_TFLOPs": round(flops / (tms / cnt / 1e3) / 1e12, 3)}
+            finals[name] = U.double()
+            del g, sol
+        ref = finals["f64"]
+
+        def errs(a):
+            return [float(f"{float(torch.linalg.vector_norm(a[f] - ref[f]) / torch.linalg.vector_norm(ref[f])):.3e}")
+                    for f in range(4)]
+
+        out[f"{nx}x{ny}"] = {"ms_per_step": {k: round(v, 4) for k, v in ms.items()},
+                             "rel_l2_vs_f64": {k: errs(finals[k]) for k in ("split_f64state", "split_f32state")},
+                             "gll_d_kernel": dk}
+    return {"workload": f"KHI 2D hybrid Fourier-x ({nx}) x LGL-y, {steps} RK4 steps, dt frozen, CFL 0.4",
+            **out}
+
+
+def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0):
+    """Poisson stream (mean `rate` req/s over `window` s) of 2D Fourier KHI runs,
+    sizes {128, 256, 512, 1024}^2 with weights 0.4/0.3/0.2/0.1, own seed each,
+    t_end ~ U[0.5, 2.0], FP32 (FP64 time, dt and sums), served with continuous
+    batching under an HBM budget; p50/p99 time-to-solution and throughput."""
+    from phrike_amd.serving import (EnsembleServer, capacities_for_budget, latency_summary,
+                                    poisson_requests)
+    sizes = {128: 0.4, 256: 0.3, 512: 0.2, 1024: 0.1}
+    caps = capacities_for_budget(sizes, int(budget_gb * 2 ** 30), "single")
+    srv = EnsembleServer(caps, precision="single", steps_per_replay=32)
+    reqs = poisson_requests(rate, window, sizes, seed=seed)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    summ = latency_summary(res, wall)
+    per = {}
+    for n in sizes:
+        lat = [r.latency for r in res if r.n == n]
+        if lat:
+            per[str(n)] = {"count": len(lat), "p50_s": round(float(np.percentile(lat, 50)), 3),
+                           "p99_s": round(float(np.percentile(lat, 99)), 3)}
+    return {"workload": f"Poisson {rate} req/s over {window} s, KHI 2D sizes 128..1024 "
+                        f"(0.4/0.3/0.2/0.1), t_end U[0.5,2], FP32",
+            "requests": len(reqs), "slots": {str(k): v for k, v in caps.items()},
+            "hbm_budget_gb": budget_gb, **{k: (round(v, 3) if isinstance(v, float) else v)
+                                            for k, v in summ.items()},
+            "offered_req_per_s": rate, "completed_req_per_s": round(len(res) / wall, 3),
+            "per_size": per}
+
+
 def load_traffic(kernel: str, n: int, prec: str):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     try:
@@ -160,6 +250,10 @@ def main():
                     help="skip the FP32-vs-FP64 turbulence study")
     ap.add_argument("--precision-size", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=200)
+    ap.add_argument("--hybrid-steps", type=int, default=1000)
+    ap.add_argument("--ensemble-window", type=float, default=3.0)
+    ap.add_argument("--ensemble-rate", type=float, default=20.0)
+    ap.add_argument("--ensemble-budget-gb", type=float, default=48.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,6 +346,15 @@ def main():
         del U, s, g
         torch.cuda.empty_cache()
         extras["fp32_vs_fp64"] = precision_study(args.precision_size, args.precision_steps, dev)
+        torch.cuda.empty_cache()
+        if args.hybrid_steps > 0:
+            print(f"[bench] hybrid KHI: {args.hybrid_steps} steps", file=sys.stderr, flush=True)
+            extras["hybrid_khi"] = hybrid_study(256, (128, 256), args.hybrid_steps, dev)
+        if args.ensemble_window > 0:
+            print(f"[bench] ensemble: {args.ensemble_rate} req/s over {args.ensemble_window} s",
+                  file=sys.stderr, flush=True)
+            extras["ensemble"] = ensemble_study(args.ensemble_rate, args.ensemble_window,
+                                                args.ensemble_budget_gb)
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu baseline: oracle TGV {args.cpu_size}^3 x {args.cpu_steps}", file=sys.stderr,

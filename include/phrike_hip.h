@@ -41,6 +41,7 @@ extern "C" {
 
 typedef struct ph_plan3d_s* ph_plan3d_t;
 typedef struct ph_plan2d_s* ph_plan2d_t;
+typedef struct ph_planh_s* ph_planh_t;
 
 /* Library ABI version (no reference counterpart). */
 int ph_abi_version(void);
@@ -135,11 +136,13 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
               void* stream);
 
 /* Per-run max of |ux| + sqrt(gamma p / rho) -> max_dev[run] and, when dt_dev
- * is not NULL, dt_dev[run] = cfl*min(dx,dy)/max (1e-6 if max <= 0).
+ * is not NULL, dt_dev[run] = cfl*min(dx,dy)/max (1e-6 if max <= 0).  Runs with
+ * active_dev[run] == 0 are not read (their outputs are meaningless);
+ * active_dev may be NULL.
  * Replaces EulerEquations2D.max_wave_speed (phrike/equations.py:365-370,
  * 137-147) and SpectralSolver2D.compute_dt (phrike/solver.py:613-620). */
 int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max_dev,
-                        double* dt_dev, double cfl, void* ws, void* stream);
+                        double* dt_dev, double cfl, const int* active_dev, void* ws, void* stream);
 
 /* Per-run FP64 sums of rho, rho*ux, rho*uy, energy -> out4_dev[run*4 + i].
  * Replaces the record() sums of SpectralSolver2D.run
@@ -169,14 +172,52 @@ int ph_runs_advance(int nbatch, double* t_dev, const double* t_end_dev, const do
  *   Dhi = fp32(D), Dlo = fp32(D - Dhi)); data float32.
  * mode PH_GLL_SPLIT_F64IO: FP32 MFMA on float64 data split as hi + lo:
  *   Dhi*Fhi + Dhi*Flo + Dlo*Fhi, result in float64.
+ * mode | PH_GLL_OFFSET (split modes): apply D to F - F[row 0] per column,
+ *   which equals D.F when the rows of D sum to zero (a derivative matrix) and
+ *   keeps FP32 rounding proportional to the variation of F along the line.
  * ny, nx multiples of 16.  Replaces LegendreLobattoBasis2D.dy /
  * LegendreLobattoBasis1D.dx applies (phrike/basis/legendre.py:787-802,
  * 1020-1028; torch matmul at 1027). */
 #define PH_GLL_F64 0
 #define PH_GLL_SPLIT_F32 1
 #define PH_GLL_SPLIT_F64IO 2
+#define PH_GLL_OFFSET 8
 int ph_gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
                  const void* F, const void* add, void* out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * 2D Euler on the hybrid grid: Fourier along x (periodic, length lx, nx
+ * points), Legendre-Gauss-Lobatto collocation along y (ny nodes).  State U is
+ * (nbatch, 4, ny, nx): rho, mx, my, E.
+ *   rhs = -(dx Fx + D Fy), dx the masked/filtered 1D spectral derivative of
+ *   Grid1D.dx1 (phrike/grid.py:512-519) per row, D the LGL derivative matrix
+ *   along y (LegendreLobattoBasis1D.D_x, phrike/basis/legendre.py:714-717);
+ *   step: RK4 with the positivity clamp (phrike/solver.py:467-513), then,
+ *   with apply_filter, x mask*sigma (Grid1D.apply_spectral_filter,
+ *   phrike/grid.py:533-538) and the LGL nodal filter along y
+ *   (phrike/basis/legendre.py:817-872);
+ *   dt = cfl * min(lx/nx, dy_min) / max(|ux| + a) (phrike/solver.py:613-620).
+ * y_bc 1 (wall): along y, the 1D Legendre grid's wall rule (phrike/grid.py:
+ *   621-738): my = 0 on the boundary rows, rho and E copied from the adjacent
+ *   row; applied to every RHS input (the step's base state included, as the
+ *   reference's in-place BC at phrike/solver.py:64-68) and after the filter.
+ *   y_bc 0: none (the reference hybrid grid's no-op, phrike/grid.py:1082-1084).
+ * D_host and Fn_host are row-major ny x ny float64 (derivative, nodal filter;
+ * pass the identity for no y filter).  gll_mode: PH_GLL_F64 or
+ * PH_GLL_SPLIT_F64IO for float64 data, PH_GLL_SPLIT_F32 for float32.
+ * nx a power of two in [16, 1024], ny a multiple of 16.  The reference's
+ * own Legendre Grid2D path has no working counterpart (Gauss nodes, FFTs
+ * across the non-periodic axis: phrike/grid.py:738-772, 852-891). */
+int ph_planh_create(ph_planh_t* plan, int nx, int ny, double lx, double dy_min, int dtype, int dealias,
+                    int filter_enabled, int filter_p, double filter_alpha, double gamma, int batch,
+                    const double* D_host, const double* Fn_host, int gll_mode, int y_bc);
+int ph_planh_destroy(ph_planh_t plan);
+int ph_planh_workspace_bytes(ph_planh_t plan, size_t* bytes);
+int ph_rhs2dh(ph_planh_t plan, int nbatch, const void* U, void* rhs, void* ws, void* stream);
+int ph_step2dh(ph_planh_t plan, int nbatch, const void* U, void* out, double dt, const double* dt_dev,
+               const int* active_dev, int apply_filter, void* ws, void* stream);
+int ph_max_wave_speed2dh(ph_planh_t plan, int nbatch, const void* U, double* max_dev, double* dt_dev,
+                         double cfl, void* ws, void* stream);
 
 #ifdef __cplusplus
 }

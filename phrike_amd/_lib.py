@@ -40,17 +40,27 @@ _SIGNATURES = {
     "ph_step2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
                           c_void_p, c_void_p]),
     "ph_max_wave_speed2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                                    c_void_p]),
+                                    c_void_p, c_void_p]),
     "ph_conserved_sums2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_runs_clip": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_runs_advance": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_gll_apply": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p]),
+    "ph_planh_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
+                                c_int, c_int, c_double, c_double, c_int, c_void_p, c_void_p, c_int, c_int]),
+    "ph_planh_destroy": (c_int, [c_void_p]),
+    "ph_planh_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_rhs2dh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_step2dh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
+                           c_void_p, c_void_p]),
+    "ph_max_wave_speed2dh": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                     c_void_p]),
 }
 
 PH_GLL_F64 = 0
 PH_GLL_SPLIT_F32 = 1
 PH_GLL_SPLIT_F64IO = 2
+PH_GLL_OFFSET = 8
 
 _lib = None
 

@@ -129,6 +129,8 @@ class LegendreLobattoBasis1D:
             m = _lib.PH_GLL_F64
         elif mode == "split":
             m = _lib.PH_GLL_SPLIT_F64IO if Fc.dtype == torch.float64 else _lib.PH_GLL_SPLIT_F32
+            if matrix == "D":  # rows sum to zero: apply to F - F[row 0]
+                m |= _lib.PH_GLL_OFFSET
         else:
             raise ValueError(mode)
         addc = add.contiguous() if add is not None else None

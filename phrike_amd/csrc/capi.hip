@@ -65,6 +65,11 @@ void with2(ph_plan2d_t p, F&& f) {
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 }  // namespace
 
+namespace ph {
+// shared with the other translation units' entry points (ph_last_error)
+void set_last_error(const char* msg) { g_err = msg; }
+}  // namespace ph
+
 extern "C" {
 
 int ph_abi_version(void) { return 1; }
@@ -250,13 +255,13 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
 }
 
 int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max_dev, double* dt_dev,
-                        double cfl, void* ws, void* stream) {
+                        double cfl, const int* active_dev, void* ws, void* stream) {
   return guard([&] {
     need(U && ws && (max_dev || dt_dev), "null buffer");
     with2(plan, [&](auto& p) {
       PH_R2;
       p.check_nb(nbatch);
-      p.max_speed(nbatch, static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, S(stream));
+      p.max_speed(nbatch, static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, S(stream), active_dev);
     });
   });
 }

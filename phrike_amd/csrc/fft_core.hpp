@@ -132,11 +132,11 @@ struct LastRegs {
 };
 
 // load the R inputs of butterfly b of pass (R, Ns) and apply the twiddles
-template <typename T, int N, int L, int DIR, int R, int NS>
+template <typename T, int N, int L, int DIR, int R, int NS, int LS = L>
 __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int b, cx<T>* v) {
   const int l = b % L, j = b / L;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = buf[(j + r * (N / R)) * L + l];
+  for (int r = 0; r < R; ++r) v[r] = buf[(j + r * (N / R)) * LS + l];
   if constexpr (NS > 1) {
     const int k = j % NS;
 #pragma unroll
@@ -149,8 +149,10 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
   dftR<R, DIR>(v);
 }
 
-// one Stockham pass, LDS -> LDS (in place; ends with a barrier)
-template <typename T, int N, int L, int DIR, int NT, int P>
+// one Stockham pass, LDS -> LDS (in place; ends with a barrier).  Lines are
+// interleaved with stride LS >= L (padding keeps 16-byte accesses free of
+// bank conflicts when L is not a multiple of 8).
+template <typename T, int N, int L, int DIR, int NT, int P, int LS = L>
 __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
   constexpr int R = Plan<N>::radix(P);
   constexpr int NS = PassNs<N, P>::value;
@@ -160,7 +162,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS>(buf, tw, b, v[q]);
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS>(buf, tw, b, v[q]);
   }
   lds_barrier();
 #pragma unroll
@@ -171,25 +173,26 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
       const int k = j % NS;
       const int d = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(d + r * NS) * L + l] = v[q][r];
+      for (int r = 0; r < R; ++r) buf[(d + r * NS) * LS + l] = v[q][r];
     }
   }
   lds_barrier();
 }
 
-template <typename T, int N, int L, int DIR, int NT, int P, int PEND>
+template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L>
 __device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
   if constexpr (P < PEND) {
-    pass_lds<T, N, L, DIR, NT, P>(buf, tw, tid);
-    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND>(buf, tw, tid);
+    pass_lds<T, N, L, DIR, NT, P, LS>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS>(buf, tw, tid);
   }
 }
 
-// Full transform of the L lines in buf, result in buf (natural order).
-// Caller must have synchronised after filling buf; ends with a barrier.
-template <typename T, int N, int L, int DIR, int NT>
+// Full transform of the L lines in buf (line stride LS), result in buf
+// (natural order).  Caller must have synchronised after filling buf; ends
+// with a barrier.
+template <typename T, int N, int L, int DIR, int NT, int LS = L>
 __device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass>(buf, tw, tid);
+  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass, LS>(buf, tw, tid);
 }
 
 // Full transform with the last pass left in registers v[q][r]; output r of

@@ -12,10 +12,18 @@
 // (checked on the device with single-entry matrices).
 //
 // Split precision: D = Dhi + Dlo with Dhi = fp32(D), Dlo = fp32(D - Dhi)
-// (built on the host from the FP64 table).  For FP32 data the product is
-// Dhi*F + Dlo*F; for FP64 data F = Fhi + Flo as well and the product is
-// Dhi*Fhi + Dhi*Flo + Dlo*Fhi, each term accumulated in its own FP32
-// register set and summed in FP64 at the end.
+// (built on the host from the FP64 table).  The operand is G = F - F[row 0]
+// per column: the rows of a collocation derivative sum to zero, so D G = D F,
+// and FP32 rounding then scales with the variation of F along the line, not
+// with its size (a uniform pressure inside a momentum flux, times entries of
+// order ny^2, would otherwise dominate the error).  For FP32 data the product
+// is Dhi*G + Dlo*G; for FP64 data G = Ghi + Glo as well and the product is
+// Dhi*Ghi + Dhi*Glo + Dlo*Ghi, each term accumulated in its own FP32 register
+// set and summed in FP64 at the end.  The filter matrix (rows summing to one)
+// is applied with offset=0.
+//
+// Each lane keeps the operands of up to KC k-steps in registers and issues
+// their loads together, so one global-load latency is paid per KC steps.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,6 +37,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int GLL_WAVES = 4;  // a block: 4 waves stacked along i (64 rows)
+
+constexpr int GLL_KC = 32;  // k-steps (of 4) whose operands are loaded together
 
 // EPI: 0 out = acc; 1 out = -(acc + add)  (hybrid RHS epilogue)
 template <int EPI>
@@ -46,10 +56,17 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_f64(const double* __rest
   d4 acc = {0.0, 0.0, 0.0, 0.0};
   const double* Drow = D + (long long)(i0 + r) * ny;
   const double* Fc = F + bo + x0 + r;
-  for (int k0 = 0; k0 < ny; k0 += 4) {
-    const double a = Drow[k0 + q];
-    const double b = Fc[(long long)(k0 + q) * nx];
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  const int nk = ny / 4;
+  for (int s0 = 0; s0 < nk; s0 += GLL_KC) {
+    double a[GLL_KC], b[GLL_KC];
+#pragma unroll
+    for (int s = 0; s < GLL_KC; ++s) {
+      const int k = 4 * (s0 + s) + q;
+      a[s] = s0 + s < nk ? Drow[k] : 0.0;
+      b[s] = s0 + s < nk ? Fc[(long long)k * nx] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < GLL_KC; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -66,7 +83,8 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_split(const float* __res
                                                              const float* __restrict__ Dlo,
                                                              const T* __restrict__ F,
                                                              const T* __restrict__ add,
-                                                             T* __restrict__ out, int ny, int nx) {
+                                                             T* __restrict__ out, int ny, int nx,
+                                                             int offset) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int i0 = (blockIdx.y * GLL_WAVES + wave) * 16;
@@ -77,16 +95,29 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_split(const float* __res
   f4 c1 = {0.f, 0.f, 0.f, 0.f}, c2 = {0.f, 0.f, 0.f, 0.f}, c3 = {0.f, 0.f, 0.f, 0.f};
   const long long drow = (long long)(i0 + r) * ny;
   const T* Fc = F + bo + x0 + r;
-  for (int k0 = 0; k0 < ny; k0 += 4) {
-    const float ah = Dhi[drow + k0 + q];
-    const float al = Dlo[drow + k0 + q];
-    const T bv = Fc[(long long)(k0 + q) * nx];
-    const float bh = float(bv);
-    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, c1, 0, 0, 0);
-    c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, c2, 0, 0, 0);
-    if constexpr (sizeof(T) == 8) {
-      const float bl = float(bv - double(bh));
-      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, c3, 0, 0, 0);
+  const T c0 = offset ? Fc[0] : T(0);  // this lane's column at row 0
+  const int nk = ny / 4;
+  for (int s0 = 0; s0 < nk; s0 += GLL_KC) {
+    float ah[GLL_KC], al[GLL_KC];
+    T bv[GLL_KC];
+#pragma unroll
+    for (int s = 0; s < GLL_KC; ++s) {
+      const int k = 4 * (s0 + s) + q;
+      const bool in = s0 + s < nk;
+      ah[s] = in ? Dhi[drow + k] : 0.f;
+      al[s] = in ? Dlo[drow + k] : 0.f;
+      bv[s] = in ? Fc[(long long)k * nx] : T(0);
+    }
+#pragma unroll
+    for (int s = 0; s < GLL_KC; ++s) {
+      const T g = bv[s] - c0;
+      const float bh = float(g);
+      c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bh, c1, 0, 0, 0);
+      c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al[s], bh, c2, 0, 0, 0);
+      if constexpr (sizeof(T) == 8) {
+        const float bl = float(g - double(bh));
+        c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[s], bl, c3, 0, 0, 0);
+      }
     }
   }
 #pragma unroll
@@ -107,6 +138,8 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
   if (ny < 16 || nx < 16 || ny % 16 || nx % 16 || nbatch < 1 || nbatch > 65535)
     throw Error(PH_ERR_ARG, "ny, nx must be positive multiples of 16; nbatch in [1, 65535]");
   if (!F || !out) throw Error(PH_ERR_ARG, "null buffer");
+  const int offset = (mode & PH_GLL_OFFSET) ? 1 : 0;
+  mode &= ~PH_GLL_OFFSET;
   dim3 block(64 * GLL_WAVES);
   dim3 grid(nx / 16, (ny + 16 * GLL_WAVES - 1) / (16 * GLL_WAVES), nbatch);
   const bool epi = add != nullptr;
@@ -125,20 +158,22 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
       if (epi)
         k_gll_split<float, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                       (const float*)F, (const float*)add, (float*)out,
-                                                      ny, nx);
+                                                      ny, nx, offset);
       else
         k_gll_split<float, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
-                                                      (const float*)F, nullptr, (float*)out, ny, nx);
+                                                      (const float*)F, nullptr, (float*)out, ny, nx,
+                                                      offset);
       break;
     case PH_GLL_SPLIT_F64IO:
       if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
       if (epi)
         k_gll_split<double, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                        (const double*)F, (const double*)add,
-                                                       (double*)out, ny, nx);
+                                                       (double*)out, ny, nx, offset);
       else
         k_gll_split<double, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
-                                                       (const double*)F, nullptr, (double*)out, ny, nx);
+                                                       (const double*)F, nullptr, (double*)out, ny, nx,
+                                                       offset);
       break;
     default:
       throw Error(PH_ERR_ARG, "unknown mode");

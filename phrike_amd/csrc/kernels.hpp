@@ -70,7 +70,19 @@ struct XFwdArgs {
   const T* sig;  // sigma_x(m) * scale, m < mx
   const T* kk;   // kx(m), zero at the Nyquist mode
   T gamma;
+  const int* active;  // per-slab flag (batched 2D) or null
 };
+
+// LDS line stride (complex values per x position) of the x kernels: by
+// default the line count itself (dense, smallest LDS footprint, which sets
+// how many blocks share a CU); PH_XLS > 0 pads every position to PH_XLS.
+#ifndef PH_XLS
+#define PH_XLS 0
+#endif
+constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : lines; }
+#ifndef PH_XMID_PREFETCH
+#define PH_XMID_PREFETCH 1
+#endif
 
 template <int MODE> struct XFwdShape;
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
@@ -135,12 +147,12 @@ __device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, 
               if (l == g + 1) v = v + p_;
             }
           }
-          bufr[2 * (x * LX + l) + line] = v;
+          bufr[2 * (x * xstride(LX) + l) + line] = v;
         }
       }
     }
     lds_barrier();
-    fft_lds<T, NX, LX, -1, NT>(buf, tw, tid);
+    fft_lds<T, NX, LX, -1, NT, xstride(LX)>(buf, tw, tid);
     // split the two real spectra, keep mx modes, scale, write two rows
     for (int i = tid; i < LX * a.mxp; i += NT) {
       const int l = i / a.mxp, k = i - l * a.mxp;
@@ -148,8 +160,8 @@ __device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, 
       if (MODE == XF_COPY && l >= ncomp) continue;
       cx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
       if (k < a.mx) {
-        const cx<T> zk = buf[k * LX + l];
-        const cx<T> zn = buf[((NX - k) & (NX - 1)) * LX + l];
+        const cx<T> zk = buf[k * xstride(LX) + l];
+        const cx<T> zn = buf[((NX - k) & (NX - 1)) * xstride(LX) + l];
         A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
         B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
         const T s = a.sig[k];
@@ -176,9 +188,10 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a
   constexpr int LX = S::LX, NC = S::NC;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   __shared__ cx<T> tw[NX];
-  __shared__ cx<T> buf[LX * NX];
+  __shared__ cx<T> buf[xstride(LX) * NX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
+  if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
   for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
 
@@ -224,13 +237,14 @@ struct ColArgs {
   const cx<T>* tw;
   const T* sig;  // by compact index
   const T* kk;   // by compact index
+  const int* active;  // per blockIdx.y flag (batched 2D runs) or null
 };
 
 #ifndef PH_COL_ELEMS_F64
 #define PH_COL_ELEMS_F64 4096
 #endif
 #ifndef PH_COL_ELEMS_F32
-#define PH_COL_ELEMS_F32 8192
+#define PH_COL_ELEMS_F32 4096
 #endif
 
 template <typename T, int N>
@@ -238,7 +252,7 @@ struct ColTile {
   static constexpr int ELEMS = (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : PH_COL_ELEMS_F32;
   static constexpr int TM = clampi(ELEMS / N, 1, 64);
   // 8 (FP64) or 16 (FP32) complex values of each register array per thread
-  static constexpr int NT = clampi((N * TM) / (sizeof(T) == 8 ? 8 : 16), 64, 512);
+  static constexpr int NT = clampi((N * TM) / 8, 64, 1024);
 };
 
 #ifndef PH_COL_WAVES_PER_EU
@@ -248,8 +262,9 @@ struct ColTile {
 // Per block, the global-source transforms run in a fixed order (field
 // sequence below); the pass-0 inputs of transform k+1 are loaded while
 // transform k runs its LDS passes.  For the two-input modes the first
-// spectrum stays in registers (Y) while the second is transformed, so every
-// output is written once.
+// spectrum waits in a second LDS array (ybuf) while the second is
+// transformed; each thread reads back only the slots it wrote, and every
+// output is written to HBM once.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
   constexpr int TM = ColTile<T, N>::TM;
@@ -259,6 +274,7 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
   constexpr int PER = LR::PER, R = LR::R;
   __shared__ cx<T> tw[N];
   __shared__ cx<T> buf[N * TM];
+  if (a.active != nullptr && a.active[blockIdx.y] == 0) return;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * TM;
   for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
@@ -269,14 +285,10 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
   const int ncol = a.ncol;
   const int nc = a.nc;
   cx<T> X[PER][R];
-  cx<T> Y[(MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV) ? PER : 1][R];  // first spectrum of a pair
   cx<T> W[P0::PER][P0::R];
-  auto copy_regs = [&](auto& dst, const auto& src) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q)
-#pragma unroll
-      for (int r = 0; r < R; ++r) dst[q][r] = src[q][r];
-  };
+  // first spectrum of a pair, at the kept positions this thread owns
+  constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
+  __shared__ cx<T> ybuf[PAIR ? N * TM : 1];
 
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
@@ -341,12 +353,14 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
     } else if constexpr (MODE == CM_FWD_DIV3) {
       // G = sigma (X + i k Z) with X = F(Fx^), Z = F(Fy^); then sigma F(Fz^)
       fwd(FWD{});
-      copy_regs(Y, X);
+      each([&](int q, int r, int l, int n, bool ok) {
+        if (kp.kept(n, N)) ybuf[n * TM + l] = X[q][r];
+      });
       fwd(FWD{});
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          oc[(long long)j * ols + l] = cscale(Y[q][r] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+          oc[(long long)j * ols + l] = cscale(ybuf[n * TM + l] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
         }
       });
       cx<T>* o2 = out + (long long)(5 + c) * a.out_fs + c0;
@@ -361,7 +375,9 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
       // CM_FWDINV_DIV / CM_FWDINV_COPY: forward, combine on kept modes, inverse
       fwd(FWD{});
       if constexpr (MODE == CM_FWDINV_DIV) {
-        copy_regs(Y, X);
+        each([&](int q, int r, int l, int n, bool ok) {
+          if (kp.kept(n, N)) ybuf[n * TM + l] = X[q][r];
+        });
         fwd(FWD{});
       }
       each([&](int q, int r, int l, int n, bool ok) {
@@ -369,7 +385,7 @@ __global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_co
         if (kp.kept(n, N)) {
           const int j = kp.compact(n, N);
           if constexpr (MODE == CM_FWDINV_DIV) {
-            v = cscale(Y[q][r] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+            v = cscale(ybuf[n * TM + l] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
           } else {
             v = cscale(X[q][r], a.sig[j]);
             if (a.ikflag) v = mul_ik(v, a.kk[j]);
@@ -449,10 +465,10 @@ __device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, cx<T>
         z = mk(A.x + B.y, B.x - A.y);
       }
     }
-    buf[n * NC + l] = z;
+    buf[n * xstride(NC) + l] = z;
   }
   lds_barrier();
-  fft_lds<T, NX, NC, +1, NT>(buf, tw, tid);
+  fft_lds<T, NX, NC, +1, NT, xstride(NC)>(buf, tw, tid);
 }
 
 // RK4 stage combine at one point (phrike/solver.py:1220-1230 3D, 502-513 2D)
@@ -461,7 +477,7 @@ __device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, cx<T>
 // clamps u (phrike/solver.py:467-488).
 template <typename T, int NC, bool CLAMP>
 __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC], long long o, double dtd,
-                                         T (&u)[NC]) {
+                                         T (&u)[NC], const T (&un)[NC], const T (&acc)[NC]) {
   const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0);
   if (a.stage < 4) {
     const T coef = (a.stage == 3) ? full : half;
@@ -469,19 +485,27 @@ __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC],
     for (int c = 0; c < NC; ++c) {
       const T k = -v[c];
       const long long idx = c * a.fs + o;
-      const T un = a.un[idx];
       if (a.stage == 1) a.out[idx] = k;
-      else a.out[idx] = a.out[idx] + T(2) * k;
-      u[c] = un + coef * k;
+      else a.out[idx] = acc[c] + T(2) * k;
+      u[c] = un[c] + coef * k;
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const long long idx = c * a.fs + o;
-      u[c] = a.un[idx] + sixth * (a.out[idx] + (-v[c]));
-    }
+    for (int c = 0; c < NC; ++c) u[c] = un[c] + sixth * (acc[c] + (-v[c]));
   }
   if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
+}
+
+template <typename T, int NC, bool CLAMP>
+__device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC], long long o, double dtd,
+                                         T (&u)[NC]) {
+  T un[NC], acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    un[c] = a.un[c * a.fs + o];
+    acc[c] = a.stage > 1 ? a.out[c * a.fs + o] : T(0);
+  }
+  rk_point<T, NC, CLAMP>(a, v, o, dtd, u, un, acc);
 }
 
 template <typename T, int NX, int NC, int MODE>
@@ -489,7 +513,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   __shared__ cx<T> tw[NX];
-  __shared__ cx<T> buf[NC * NX];
+  __shared__ cx<T> buf[xstride(NC) * NX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
@@ -508,7 +532,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
     const long long o = base + p;  // rows y0, y0+1 are contiguous
     T v[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * NC + c) + line];
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * xstride(NC) + c) + line];
     if constexpr (MODE == XI_OUT) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
@@ -544,19 +568,37 @@ template <typename T, int NX, int NC, int FMODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m) {
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
-  constexpr int LB = NC > XFwdShape<FMODE>::LX ? NC : XFwdShape<FMODE>::LX;
   static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
   __shared__ cx<T> tw[NX];
-  __shared__ cx<T> buf[(LB > 5 ? LB : 5) * NX];
+  // inverse (NC lines), forward (FMODE lines) and the stage-4 copy (5 lines)
+  constexpr int LB0 = xstride(NC) > xstride(XFwdShape<FMODE>::LX) ? xstride(NC) : xstride(XFwdShape<FMODE>::LX);
+  constexpr int LB = LB0 > xstride(XFwdShape<XF_COPY>::LX) ? LB0 : xstride(XFwdShape<XF_COPY>::LX);
+  __shared__ cx<T> buf[LB * NX];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
   for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
+  // with PH_XMID_PREFETCH the RK combine's inputs are loaded before the
+  // inverse transform so their latency overlaps it (more live registers)
+  T pre_un[PH_XMID_PREFETCH ? PPT : 1][NC], pre_acc[PH_XMID_PREFETCH ? PPT : 1][NC];
+#pragma unroll
+  for (int pi = 0; pi < (PH_XMID_PREFETCH ? PPT : 0); ++pi) {
+    const int p = tid + pi * NT;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      pre_un[pi][c] = T(0);
+      pre_acc[pi][c] = T(0);
+      if (p < 2 * NX) {
+        pre_un[pi][c] = a.un[c * a.fs + base + p];
+        if (a.stage > 1) pre_acc[pi][c] = a.out[c * a.fs + base + p];
+      }
+    }
+  }
   xinv_to_lds<T, NX, NC, NT>(a, NC, tw, buf, tid, slab, y0);
   const T* bufr = reinterpret_cast<const T*>(buf);
-  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
   const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
   T q[PPT][5];
 #pragma unroll
@@ -569,8 +611,9 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
     const long long o = base + p;
     T v[NC], u[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * NC + c) + line];
-    rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * xstride(NC) + c) + line];
+    if constexpr (PH_XMID_PREFETCH) rk_point<T, NC, NC == 4>(a, v, o, dtd, u, pre_un[pi], pre_acc[pi]);
+    else rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
     if (a.stage == 4) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
@@ -626,12 +669,14 @@ __device__ __forceinline__ double dmax_nan(double a, double b) {
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long fs, long long ss,
-                                                         long long n, T gamma, double* part) {
+                                                         long long n, T gamma, double* part,
+                                                         const int* active) {
   // grid: (blocks per slab, slabs); part[slab * gridDim.x + blockIdx.x]
   __shared__ double red[NT];
   const T* us = u + (long long)blockIdx.y * ss;
   double m = 0.0;
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
+  const bool on = active == nullptr || active[blockIdx.y] != 0;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; on && i < n; i += (long long)gridDim.x * NT)
     m = dmax_nan(m, wave_speed<T, DIM>(us, fs, i, gamma));
   red[threadIdx.x] = m;
   lds_barrier();

@@ -463,7 +463,8 @@ struct Plan3D : PlanBase {
   void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    hipLaunchKernelGGL((k_maxspeed_partial<T, 3>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_maxspeed_partial<T, 3>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma), part,
+                       (const int*)nullptr);
     check_launch("k_maxspeed_partial");
     const double h = std::min(lx / nx, std::min(ly / ny, lz / nz));
     hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out, dt_out, cfl * h);
@@ -544,8 +545,10 @@ struct Plan2D : PlanBase {
     a.gamma = T(gamma);
     return a;
   }
-  void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st) const {
+  void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st,
+            const int* active = nullptr) const {
     XFwdArgs<T> a{};
+    a.active = active;
     a.in = in;
     a.in_fs = ncell;
     a.in_ss = 4 * ncell;
@@ -564,8 +567,9 @@ struct Plan2D : PlanBase {
     else launch_xfwd<T, XF_COPY>(nx, ny, nb, a, st);
     check_launch("k_xfwd(2d)");
   }
-  void ypass(bool div, int ikflag, int nb, void* ws, hipStream_t st) const {
+  void ypass(bool div, int ikflag, int nb, void* ws, hipStream_t st, const int* active = nullptr) const {
     ColArgs<T> a{};
+    a.active = active;
     a.in = A(ws);
     a.in_fs = (long long)ny * mxp;
     a.in_bs = 8ll * ny * mxp;
@@ -601,8 +605,8 @@ struct Plan2D : PlanBase {
     return a;
   }
   void deriv(int axis, int nb, const T* in, T* out, const int* active, void* ws, hipStream_t st) const {
-    xfwd(XF_COPY, in, nb, axis == 0 ? 0xFu : 0u, ws, st);
-    ypass(false, axis == 1, nb, ws, st);
+    xfwd(XF_COPY, in, nb, axis == 0 ? 0xFu : 0u, ws, st, active);
+    ypass(false, axis == 1, nb, ws, st, active);
     XInvArgs<T> a = xinv_args(ws);
     a.nf = 4;
     a.out = out;
@@ -624,9 +628,9 @@ struct Plan2D : PlanBase {
   // boundaries fused as in the 3D step
   void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active,
             bool filter, void* ws, hipStream_t st) const {
-    xfwd(XF_FLUX2, U, nb, 0xFu, ws, st);
+    xfwd(XF_FLUX2, U, nb, 0xFu, ws, st, active);
     for (int s = 1; s <= 4; ++s) {
-      ypass(true, 0, nb, ws, st);
+      ypass(true, 0, nb, ws, st, active);
       XMidArgs<T> m{};
       m.inv = xinv_args(ws);
       m.inv.nf = 4;
@@ -646,7 +650,7 @@ struct Plan2D : PlanBase {
       launch_xmid<T, 4, XF_FLUX2>(nx, ny, nb, m, st);
       check_launch("k_xmid(2d)");
     }
-    ypass(false, 0, nb, ws, st);
+    ypass(false, 0, nb, ws, st, active);
     XInvArgs<T> a = xinv_args(ws);
     a.nf = 4;
     a.out = out;
@@ -655,10 +659,11 @@ struct Plan2D : PlanBase {
     check_launch("k_xinv(2d filter)");
   }
   void max_speed(int nb, const T* U, double* out, double* dt_out, double cfl, void* ws,
-                 hipStream_t st) const {
+                 hipStream_t st, const int* active = nullptr) const {
     double* part = Part(ws);
     const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
-    hipLaunchKernelGGL((k_maxspeed_partial<T, 2>), dim3(npb, nb), dim3(NT), 0, st, U, ncell, 4 * ncell, ncell, T(gamma), part);
+    hipLaunchKernelGGL((k_maxspeed_partial<T, 2>), dim3(npb, nb), dim3(NT), 0, st, U, ncell, 4 * ncell, ncell, T(gamma), part,
+                       active);
     check_launch("k_maxspeed_partial(2d)");
     const double h = std::min(lx / nx, ly / ny);
     hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out, dt_out, cfl * h);

@@ -211,7 +211,7 @@ class SpectralSolver2D:
         Ub = self._runs(U)
         out = torch.empty(Ub.shape[0], dtype=torch.float64, device=self.grid.device)
         _lib.call("ph_max_wave_speed2d", plan, Ub.shape[0], _lib.ptr(Ub), _lib.ptr(out), None,
-                  self.cfl, ws, st)
+                  self.cfl, None, ws, st)
         return out
 
     def dt_runs(self, U, out=None) -> torch.Tensor:
@@ -221,7 +221,7 @@ class SpectralSolver2D:
         if out is None:
             out = torch.empty(Ub.shape[0], dtype=torch.float64, device=self.grid.device)
         _lib.call("ph_max_wave_speed2d", plan, Ub.shape[0], _lib.ptr(Ub), None, _lib.ptr(out),
-                  self.cfl, ws, st)
+                  self.cfl, None, ws, st)
         return out
 
     def compute_dt(self, U) -> float:

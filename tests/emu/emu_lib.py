@@ -92,3 +92,44 @@ class Emu2D:
         nb = U.shape[0] if U.ndim == 4 else 1
         self._chk(self.lib.ph_rhs2d(self.h, nb, P(U), P(out), P(self.ws), None))
         return out
+
+
+class EmuHybrid:
+    def __init__(self, lib, nx, ny, D, Fn, dy_min, batch=1, lx=1.0, dtype=np.float64, dealias=True,
+                 filt=(1, 16, 36.0), gamma=1.4, mode=0, ybc=1):
+        self.lib, self.dtype = lib, dtype
+        self.D = np.ascontiguousarray(D, dtype=np.float64)
+        self.Fn = np.ascontiguousarray(Fn, dtype=np.float64)
+        h = ctypes.c_void_p()
+        code = 0 if dtype == np.float64 else 1
+        self._chk(lib.ph_planh_create(ctypes.byref(h), nx, ny, lx, dy_min, code, int(dealias), *filt, gamma,
+                                      batch, self.D.ctypes.data, self.Fn.ctypes.data, mode, ybc))
+        self.h = h
+        n = ctypes.c_size_t()
+        self._chk(lib.ph_planh_workspace_bytes(h, ctypes.byref(n)))
+        self.ws = np.zeros(n.value // 8 + 64, dtype=np.float64)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.ph_last_error().decode())
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        self._chk(self.lib.ph_rhs2dh(self.h, nb, P(U), P(out), P(self.ws), None))
+        return out
+
+    def step(self, U, dt, filt=True):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        self._chk(self.lib.ph_step2dh(self.h, nb, P(U), P(out), dt, None, None, int(filt), P(self.ws), None))
+        return out
+
+    def dt(self, U, cfl):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        out = np.zeros(nb, dtype=np.float64)
+        self._chk(self.lib.ph_max_wave_speed2dh(self.h, nb, P(U), None, P(out), cfl, P(self.ws), None))
+        return out
