@@ -50,12 +50,27 @@ def byte_model(n: int):
     return per, C
 
 
+def kernel_symbol(tag: str, n: int, prec: str) -> str:
+    """The HIP kernel (as rocprofv3 names it) that a timing tag launches."""
+    T = "double" if prec == "double" else "float"
+    return {"xfwd_flux": f"k_xfwd<{T}, {n}, 0>", "xfwd_copy": f"k_xfwd<{T}, {n}, 2>",
+            "yfwd_div": f"k_col<{T}, {n}, 0>", "yfwd_copy": f"k_col<{T}, {n}, 1>",
+            "z_div": f"k_col<{T}, {n}, 2>", "z_copy": f"k_col<{T}, {n}, 3>",
+            "yinv": f"k_col<{T}, {n}, 4>", "xinv_out": f"k_xinv<{T}, {n}, 5, 0>",
+            # one kernel for every stage boundary (stage 4 branches at run time)
+            "xmid1": f"k_xmid<{T}, {n}, 5, 0>", "xmid23": f"k_xmid<{T}, {n}, 5, 0>",
+            "xmid_last": f"k_xmid<{T}, {n}, 5, 0>"}.get(tag, tag)
+
+
+# launches of each timing tag in one RK4 step of the fused 3D pipeline
+STEP_LAUNCHES = {"xfwd_flux": 1, "yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
+                 "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xinv_out": 1}
+
+
 def fused_step_units(n: int) -> float:
     """Bytes per step (units of s*N^3) that the fused pipeline moves."""
     per, _ = byte_model(n)
-    return (per["xfwd_flux"] + 4 * (per["yfwd_div"] + per["z_div"] + per["yinv"])
-            + per["xmid1"] + 2 * per["xmid23"] + per["xmid_last"]
-            + per["yfwd_copy"] + per["z_copy"] + per["yinv"] + per["xinv_out"])
+    return sum(per[t] * k for t, k in STEP_LAUNCHES.items())
 
 
 def cpu_baseline(n: int, steps: int):
@@ -139,8 +154,10 @@ def precision_study(n: int, steps: int, dev):
 def hybrid_study(nx: int, nys, steps: int, dev):
     """2D Kelvin-Helmholtz on the hybrid grid (Fourier x on [0,1), LGL y on
     [-1/2, 1/2]): shear layers at y = +-1/4 of width 0.05, rho 1 -> 2, p = 2.5,
-    v = 0.01 sin(4 pi x) + 1e-4 N(0,1) (seed 7); `steps` RK4 steps with dt
-    frozen from CFL 0.4 at t = 0, post-step x/y filters.  FP64 with the FP64
+    v = 0.01 sin(4 pi x) + 1e-4 N(0,1) (seed 7), walls at y = +-1/2; `steps`
+    RK4 steps with dt frozen from CFL 0.3 at t = 0 (CFL 0.4 with this filter
+    turns unstable at the walls after ~850 steps at 256 x 128), post-step
+    x/y filters.  FP64 with the FP64
     matrix-core D, then the same with split-precision D (FP32 matrix cores on
     hi/lo parts of D and of the FP64 data) and with an FP32 state; reports
     per-field relative L2 against FP64, time per step, and the D.U kernel
@@ -159,7 +176,7 @@ def hybrid_study(nx: int, nys, steps: int, dev):
         for name, prec, gll in (("f64", "double", "f64"), ("split_f64state", "double", "split"),
                                 ("split_f32state", "single", "split")):
             g = HybridGrid2D(nx, ny, filter_params=fp, precision=prec, gll=gll, torch_device=str(dev))
-            sol = HybridSolver2D(g, EulerEquations2D(1.4), cfl=0.4)
+            sol = HybridSolver2D(g, EulerEquations2D(1.4), cfl=0.3)
             U0 = torch.as_tensor(khi_2d_gll(nx, g.y_nodes, seed=7), device=dev).to(g.dtype)
             if dt is None:
                 dt = sol.compute_dt(U0)
@@ -189,10 +206,16 @@ def hybrid_study(nx: int, nys, steps: int, dev):
             return [float(f"{float(torch.linalg.vector_norm(a[f] - ref[f]) / torch.linalg.vector_norm(ref[f])):.3e}")
                     for f in range(4)]
 
+        mom = max(float(torch.linalg.vector_norm(ref[1])), float(torch.linalg.vector_norm(ref[2])))
         out[f"{nx}x{ny}"] = {"ms_per_step": {k: round(v, 4) for k, v in ms.items()},
+                             "finite": bool(torch.isfinite(ref).all()),
                              "rel_l2_vs_f64": {k: errs(finals[k]) for k in ("split_f64state", "split_f32state")},
+                             "my_err_on_momentum_scale": {
+                                 k: float(f"{float(torch.linalg.vector_norm(finals[k][2] - ref[2])) / mom:.3e}")
+                                 for k in ("split_f64state", "split_f32state")},
                              "gll_d_kernel": dk}
-    return {"workload": f"KHI 2D hybrid Fourier-x ({nx}) x LGL-y, {steps} RK4 steps, dt frozen, CFL 0.4",
+    return {"workload": f"KHI 2D hybrid Fourier-x ({nx}) x LGL-y with walls, {steps} RK4 steps, "
+                        f"dt frozen, CFL 0.3",
             **out}
 
 
@@ -227,12 +250,72 @@ def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0):
 
 
 def load_traffic(kernel: str, n: int, prec: str):
+    """Measured HBM bytes per launch of `kernel` (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, scaled by a plain copy's readings), from profiles/."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     try:
         d = json.load(open(path))
-        return d.get(f"{kernel}@{n}^3/{prec}")
+        return d["kernels"].get(f"{kernel}@{n}^3")
     except Exception:
         return None
+
+
+def _pmc_per_dispatch(path: str, counter: str):
+    """{dispatch id: (kernel symbol, value)} from a rocprofv3 --pmc CSV."""
+    import csv
+    import re
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            m = re.search(r"(k_\w+<[^>]*>)", r["Kernel_Name"])
+            name = m.group(1) if m else r["Kernel_Name"]
+            did = int(r["Dispatch_Id"])
+            prev = out.get(did, (name, 0.0))[1]
+            out[did] = (name, prev + float(r["Counter_Value"]))
+    return out
+
+
+def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, n, prec, command):
+    """HBM bytes per launch of each pipeline kernel from two rocprofv3 passes
+    (FETCH_SIZE, WRITE_SIZE: KiB per dispatch), each scaled by what the same
+    counter reads for a plain device copy of known size (--calibrate-copy)."""
+    cal_bytes = (1 << 27) * 8
+
+    def factor(path, counter):
+        # the five 1 GiB copies are the largest dispatches of the calibration run
+        vals = sorted((v for _, v in _pmc_per_dispatch(path, counter).values()), reverse=True)[:5]
+        med = vals[len(vals) // 2]
+        return cal_bytes / (med * 1024.0)
+
+    ff = factor(cal_fetch_csv, "FETCH_SIZE")
+    fw = factor(cal_write_csv, "WRITE_SIZE")
+    fe = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
+    wr = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
+    es = 8 if prec == "double" else 4
+    per, _ = byte_model(n)
+    alg = {}
+    for tag, units in per.items():
+        alg.setdefault(kernel_symbol(tag, n, prec), []).extend([units * es * n ** 3] * STEP_LAUNCHES.get(tag, 1))
+    acc = {}
+    for did, (name, v) in fe.items():
+        acc.setdefault(name, [0, 0.0, 0.0])
+        acc[name][0] += 1
+        acc[name][1] += v * 1024.0 * ff
+    for did, (name, v) in wr.items():
+        if name in acc:
+            acc[name][2] += v * 1024.0 * fw
+    kern = {}
+    for name, (cnt, rd, wt) in acc.items():
+        if not name.startswith("k_"):
+            continue
+        a = alg.get(name)
+        kern[f"{name}@{n}^3"] = {"launches": cnt, "read_bytes": int(rd / cnt), "write_bytes": int(wt / cnt),
+                                 "bytes_per_launch": int((rd + wt) / cnt),
+                                 "alg_bytes_per_launch": int(sum(a) / len(a)) if a else None}
+    return {"precision": prec, "size": n, "fetch_factor": round(ff, 4), "write_factor": round(fw, 4),
+            "command": command, "kernels": kern}
 
 
 def main():
@@ -256,7 +339,27 @@ def main():
     ap.add_argument("--ensemble-budget-gb", type=float, default=48.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (HBM counter calibration)")
+    ap.add_argument("--traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV"),
+                    help="only build profiles/pmc_traffic.json from rocprofv3 --pmc CSVs of this "
+                         "benchmark (FETCH_SIZE, WRITE_SIZE) and of --calibrate-copy")
     args = ap.parse_args()
+    if args.traffic:
+        tab = traffic_table(*args.traffic, args.size, args.precision,
+                            command="rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -f csv -- python3 bench.py "
+                                    f"--size {args.size} --precision {args.precision} --steps 2 --warmup 0 "
+                                    "--no-cpu --no-extras --no-kernel-timing; and -- python3 bench.py "
+                                    "--calibrate-copy")
+        out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+        old = {}
+        if os.path.exists(out):
+            with open(out) as f:
+                old = json.load(f)
+        merged = {"kernels": {**old.get("kernels", {}), **tab.pop("kernels")},
+                  "runs": {**old.get("runs", {}), f"{args.size}^3/{args.precision}": tab}}
+        with open(out, "w") as f:
+            json.dump(merged, f, indent=1)
+        print(json.dumps(tab))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the 3D pseudo-spectral step is not sharded "
               "across GPUs in this backend (run with --gpus 1)", file=sys.stderr)
@@ -329,14 +432,26 @@ def main():
                         "alg_GBs": round(alg / (avg_ms / 1e3) / 1e9, 1) if alg else None}
         if dominant is None or ms > kt[dominant][1]:
             dominant = tag
+    # group the tags by HIP kernel symbol (xmid1 and xmid23 are one kernel),
+    # so the average launch duration is the one rocprofv3 reports for it
+    groups = {}
+    for tag, (cnt, ms) in kt.items():
+        if tag not in per:
+            continue
+        g_ = groups.setdefault(kernel_symbol(tag, n, args.precision), [0, 0.0, 0.0])
+        g_[0] += cnt
+        g_[1] += ms
+        g_[2] += cnt * per[tag] * es * n ** 3
     roof = None
-    if dominant is not None:
-        cnt, ms = kt[dominant]
-        alg = per[dominant] * es * n ** 3
+    if groups:
+        sym = max(groups, key=lambda k: groups[k][1])
+        cnt, ms, alg_tot = groups[sym]
+        alg = alg_tot / cnt
         ach = alg / (ms / cnt / 1e3)
         roof = {"bound": "hbm", "achieved": round(ach / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK, 4), "traffic": load_traffic(dominant, n, args.precision),
-                "kernel": dominant, "alg_bytes_per_launch": int(alg)}
+                "frac": round(ach / HBM_PEAK, 4), "traffic": load_traffic(sym, n, args.precision),
+                "kernel": sym, "launches": cnt, "avg_ms": round(ms / cnt, 4),
+                "alg_bytes_per_launch": int(alg)}
     sums = s.conserved_sums(U)
     del out
     extras = {}

@@ -246,10 +246,17 @@ struct ColArgs {
 #ifndef PH_COL_ELEMS_F32
 #define PH_COL_ELEMS_F32 4096
 #endif
+// FP32 modes with one transform per field (no second spectrum held in LDS)
+// take twice the columns, so a row segment is 128 bytes
+#ifndef PH_COL_ELEMS_F32_SINGLE
+#define PH_COL_ELEMS_F32_SINGLE 8192
+#endif
 
-template <typename T, int N>
+template <typename T, int N, int MODE>
 struct ColTile {
-  static constexpr int ELEMS = (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : PH_COL_ELEMS_F32;
+  static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
+  static constexpr int ELEMS =
+      (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : (PAIR ? PH_COL_ELEMS_F32 : PH_COL_ELEMS_F32_SINGLE);
   static constexpr int TM = clampi(ELEMS / N, 1, 64);
   // 8 (FP64) or 16 (FP32) complex values of each register array per thread
   static constexpr int NT = clampi((N * TM) / 8, 64, 1024);
@@ -266,9 +273,9 @@ struct ColTile {
 // transformed; each thread reads back only the slots it wrote, and every
 // output is written to HBM once.
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__((ColTile<T, N>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
-  constexpr int TM = ColTile<T, N>::TM;
-  constexpr int NT = ColTile<T, N>::NT;
+__global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
+  constexpr int TM = ColTile<T, N, MODE>::TM;
+  constexpr int NT = ColTile<T, N, MODE>::NT;
   using LR = LastRegs<N, TM, NT>;
   using P0 = P0Shape<N, TM, NT>;
   constexpr int PER = LR::PER, R = LR::R;

@@ -129,9 +129,9 @@ void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t
 template <typename T, int MODE>
 void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
   PH_SWITCH_N(n, ({
-    constexpr int TM = ColTile<T, N_>::TM;
+    constexpr int TM = ColTile<T, N_, MODE>::TM;
     dim3 grid((a.ncol + TM - 1) / TM, nblocky);
-    hipLaunchKernelGGL((k_col<T, N_, MODE>), grid, dim3(ColTile<T, N_>::NT), 0, st, a);
+    hipLaunchKernelGGL((k_col<T, N_, MODE>), grid, dim3(ColTile<T, N_, MODE>::NT), 0, st, a);
   }));
 }
 

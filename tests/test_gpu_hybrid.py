@@ -82,8 +82,9 @@ def test_hybrid_split_precision(dev):
     ec2 = np.linalg.norm(c[2] - a[2]) / mom
     print(f"hybrid {nx}x{ny} {n} steps: split(f64 state) {eb} (my/momentum {eb2:.2e}) "
           f"f32 {ec} (my/momentum {ec2:.2e})", flush=True)
-    assert eb.max() < 1e-4 and eb2 < 1e-5
-    assert ec.max() < 1e-3 and ec2 < 1e-5
+    others = [0, 1, 3]
+    assert eb[others].max() < 1e-5 and eb2 < 1e-5
+    assert ec[others].max() < 1e-5 and ec2 < 1e-5
 
 
 def test_hybrid_rejects_bad_shapes(dev):
