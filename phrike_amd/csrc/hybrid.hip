@@ -67,14 +67,14 @@ template <typename T, int NX, int MODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
-  __shared__ cx<T> tw[NX];
+  PH_XTW_DECL(T, NX);
   __shared__ cx<T> buf[xstride(4) * NX];
   T* bufr = reinterpret_cast<T*>(buf);
   const int tid = threadIdx.x;
   const int run = blockIdx.y;
   if (a.active != nullptr && a.active[run] == 0) return;
   const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  PH_XTW_FILL(a.tw, NX, NT, tid);
   const long long base = (long long)run * a.ss + (long long)y0 * NX;
 #pragma unroll
   for (int pi = 0; pi < PPT; ++pi) {

@@ -80,6 +80,21 @@ struct XFwdArgs {
 #define PH_XLS 0
 #endif
 constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : lines; }
+
+// Twiddles of the x kernels: an LDS copy per block (PH_XTW_GLOBAL 0), or read
+// where they lie in global memory through the L1 cache (1), which takes the
+// table out of LDS and lets one more block share a CU.
+#ifndef PH_XTW_GLOBAL
+#define PH_XTW_GLOBAL 0
+#endif
+#if PH_XTW_GLOBAL
+#define PH_XTW_DECL(T, NX) const cx<T>* tw = nullptr
+#define PH_XTW_FILL(src, NX, NT, tid) tw = (src)
+#else
+#define PH_XTW_DECL(T, NX) __shared__ cx<T> tw[NX]
+#define PH_XTW_FILL(src, NX, NT, tid) \
+  for (int i_ = (tid); i_ < (NX); i_ += (NT)) tw[i_] = (src)[i_]
+#endif
 #ifndef PH_XMID_PREFETCH
 #define PH_XMID_PREFETCH 1
 #endif
@@ -93,7 +108,7 @@ template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG
 // x transform of two rows at once -> kept modes to a.out.  Shared by k_xfwd
 // and the fused k_xmid.
 template <typename T, int NX, int MODE, int NT, int PPT>
-__device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, cx<T>* tw, cx<T>* buf,
+__device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
                                                int tid, int slab, int y0,
                                                const T (&q)[PPT][XFwdShape<MODE>::NC]) {
   using S = XFwdShape<MODE>;
@@ -187,13 +202,13 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a
   using S = XFwdShape<MODE>;
   constexpr int LX = S::LX, NC = S::NC;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
-  __shared__ cx<T> tw[NX];
+  PH_XTW_DECL(T, NX);
   __shared__ cx<T> buf[xstride(LX) * NX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  PH_XTW_FILL(a.tw, NX, NT, tid);
 
   const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
   const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
@@ -455,7 +470,7 @@ __device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
 // Rebuild z = row0 + i*row1 from the kept half spectra of both rows of nfl
 // fields and inverse-transform along x; result in buf (natural order).
 template <typename T, int NX, int NC, int NT>
-__device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, cx<T>* tw, cx<T>* buf, int tid,
+__device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, const cx<T>* tw, cx<T>* buf, int tid,
                                             int slab, int y0) {
   const cx<T>* src = a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp;
   for (int i = tid; i < NC * NX; i += NT) {
@@ -519,13 +534,13 @@ template <typename T, int NX, int NC, int MODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a) {
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
-  __shared__ cx<T> tw[NX];
+  PH_XTW_DECL(T, NX);
   __shared__ cx<T> buf[xstride(NC) * NX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  PH_XTW_FILL(a.tw, NX, NT, tid);
   const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
   xinv_to_lds<T, NX, NC, NT>(a, nfl, tw, buf, tid, slab, y0);
   const T* bufr = reinterpret_cast<const T*>(buf);
@@ -576,7 +591,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
-  __shared__ cx<T> tw[NX];
+  PH_XTW_DECL(T, NX);
   // inverse (NC lines), forward (FMODE lines) and the stage-4 copy (5 lines)
   constexpr int LB0 = xstride(NC) > xstride(XFwdShape<FMODE>::LX) ? xstride(NC) : xstride(XFwdShape<FMODE>::LX);
   constexpr int LB = LB0 > xstride(XFwdShape<XF_COPY>::LX) ? LB0 : xstride(XFwdShape<XF_COPY>::LX);
@@ -586,7 +601,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
-  for (int i = tid; i < NX; i += NT) tw[i] = a.tw[i];
+  PH_XTW_FILL(a.tw, NX, NT, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
   // with PH_XMID_PREFETCH the RK combine's inputs are loaded before the
   // inverse transform so their latency overlaps it (more live registers)
