@@ -42,7 +42,7 @@ def byte_model(n: int):
         "z_copy": 10 * b, "xinv_out": 5 * a + 5,
         # fused stage boundaries: D in, U in, accumulator, next-stage spectra out
         "xmid1": 5 * a + 5 + 5 + 15 * a, "xmid23": 5 * a + 5 + 10 + 15 * a,
-        "xmid_last": 5 * a + 5 + 5 + 5 + 5 * a,
+        "xmid_last": 5 * a + 5 + 5 + 5 * a,
     }
     # per-step model of the unfused pencil pipeline (stage states written and
     # re-read), the official figure for hbm_frac_step

@@ -35,6 +35,23 @@ struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 
+// Two FP32 lanes handled as one element: the column passes transform two
+// adjacent FP32 columns per slot (cx<f2> = re0 re1 im0 im1, 16 bytes), which
+// gives FP32 tiles the FP64 shape (128-byte rows at 8 slots, the same
+// registers per thread).  Scalars (twiddles, sigma, k) are splatted.
+struct alignas(8) f2 {
+  float a, b;
+  f2() = default;
+  __host__ __device__ constexpr f2(float a_, float b_) : a(a_), b(b_) {}
+  __host__ __device__ constexpr f2(double v) : a(float(v)), b(float(v)) {}
+  __host__ __device__ constexpr f2(float v) : a(v), b(v) {}
+  __host__ __device__ constexpr f2(int v) : a(float(v)), b(float(v)) {}
+};
+__device__ __forceinline__ f2 operator+(f2 u, f2 v) { return f2(u.a + v.a, u.b + v.b); }
+__device__ __forceinline__ f2 operator-(f2 u, f2 v) { return f2(u.a - v.a, u.b - v.b); }
+__device__ __forceinline__ f2 operator*(f2 u, f2 v) { return f2(u.a * v.a, u.b * v.b); }
+__device__ __forceinline__ f2 operator-(f2 u) { return f2(-u.a, -u.b); }
+
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
 template <typename T>

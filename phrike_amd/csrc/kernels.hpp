@@ -34,6 +34,46 @@ constexpr int NT = 256;  // reductions and small helpers
 
 constexpr int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Element type of the column passes: FP64 complex, or (PH_COL_PAIR_F32) two
+// adjacent FP32 columns per slot.  FP32 spectral rows are then stored
+// pair-planar: columns 2m, 2m+1 as re(2m) re(2m+1) im(2m) im(2m+1).
+#ifndef PH_COL_PAIR_F32
+#define PH_COL_PAIR_F32 1
+#endif
+template <typename T>
+struct ColElem {
+  using type = T;
+  static constexpr int PW = 1;
+};
+#if PH_COL_PAIR_F32
+template <>
+struct ColElem<float> {
+  using type = f2;
+  static constexpr int PW = 2;
+};
+#endif
+
+// spectral row element k (x kernels), in the layout of ColElem
+template <typename T>
+__device__ __forceinline__ cx<T> spec_ld(const cx<T>* row, int k) {
+  if constexpr (ColElem<T>::PW == 2) {
+    const T* f = reinterpret_cast<const T*>(row) + (k >> 1) * 4 + (k & 1);
+    return mk(f[0], f[2]);
+  } else {
+    return row[k];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void spec_st(cx<T>* row, int k, cx<T> v) {
+  if constexpr (ColElem<T>::PW == 2) {
+    T* f = reinterpret_cast<T*>(row) + (k >> 1) * 4 + (k & 1);
+    f[0] = v.x;
+    f[2] = v.y;
+  } else {
+    row[k] = v;
+  }
+}
+
 // Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
 // of the block's 2*NX points, which bounds the per-thread flux and FFT state.
 template <typename T, int NX>
@@ -188,9 +228,9 @@ __device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, 
           B = mul_ik(B, kx);
         }
       }
-      cx<T>* o = dst + (long long)f * a.out_fs + k;
-      o[0] = A;
-      o[a.mxp] = B;
+      cx<T>* row = dst + (long long)f * a.out_fs;
+      spec_st(row, k, A);
+      spec_st(row + a.mxp, k, B);
     }
     lds_barrier();
   }
@@ -267,14 +307,29 @@ struct ColArgs {
 #define PH_COL_ELEMS_F32_SINGLE 8192
 #endif
 
+// Two-input modes hold the first spectrum in registers (YREG) or in a second
+// LDS array (default: measured equal in FP32 at 8-column tiles, and the
+// registers spill in FP64 or at 16 FP32 columns).
+#ifndef PH_COL_PER_THREAD_F32
+#define PH_COL_PER_THREAD_F32 8
+#endif
+#ifndef PH_COL_YREG_F32
+#define PH_COL_YREG_F32 0
+#endif
+#ifndef PH_COL_YREG_F64
+#define PH_COL_YREG_F64 0
+#endif
+
 template <typename T, int N, int MODE>
 struct ColTile {
   static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
+  static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
   static constexpr int ELEMS =
       (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : (PAIR ? PH_COL_ELEMS_F32 : PH_COL_ELEMS_F32_SINGLE);
   static constexpr int TM = clampi(ELEMS / N, 1, 64);
-  // 8 (FP64) or 16 (FP32) complex values of each register array per thread
-  static constexpr int NT = clampi((N * TM) / 8, 64, 1024);
+  // complex values of each register array per thread
+  static constexpr int PERT = (sizeof(T) == 8) ? 8 : PH_COL_PER_THREAD_F32;
+  static constexpr int NT = clampi((N * TM) / PERT, 64, 1024);
 };
 
 #ifndef PH_COL_WAVES_PER_EU
@@ -284,9 +339,9 @@ struct ColTile {
 // Per block, the global-source transforms run in a fixed order (field
 // sequence below); the pass-0 inputs of transform k+1 are loaded while
 // transform k runs its LDS passes.  For the two-input modes the first
-// spectrum waits in a second LDS array (ybuf) while the second is
-// transformed; each thread reads back only the slots it wrote, and every
-// output is written to HBM once.
+// spectrum waits in registers (Y) or in a second LDS array (ybuf) while the
+// second is transformed; each thread reads back only the slots it wrote, and
+// every output is written to HBM once.
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) void k_col(ColArgs<T> a) {
   constexpr int TM = ColTile<T, N, MODE>::TM;
@@ -309,8 +364,14 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   cx<T> X[PER][R];
   cx<T> W[P0::PER][P0::R];
   // first spectrum of a pair, at the kept positions this thread owns
-  constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
-  __shared__ cx<T> ybuf[PAIR ? N * TM : 1];
+  constexpr bool PAIR = ColTile<T, N, MODE>::PAIR;
+  constexpr bool YREG = ColTile<T, N, MODE>::YREG;
+  __shared__ cx<T> ybuf[(PAIR && !YREG) ? N * TM : 1];
+  cx<T> Y[YREG ? PER : 1][R];
+  auto first = [&](int q, int r, int l, int n) {
+    if constexpr (YREG) return Y[q][r];
+    else return ybuf[n * TM + l];
+  };
 
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
@@ -342,6 +403,12 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         f(q, r, l, n, c0 + l < ncol);
       }
     }
+  };
+  auto keep_first = [&]() {
+    each([&](int q, int r, int l, int n, bool ok) {
+      if constexpr (YREG) Y[q][r] = X[q][r];
+      else if (kp.kept(n, N)) ybuf[n * TM + l] = X[q][r];
+    });
   };
   int k = 0;
   auto fwd = [&](auto dir) {  // next global-source transform into X
@@ -375,14 +442,12 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     } else if constexpr (MODE == CM_FWD_DIV3) {
       // G = sigma (X + i k Z) with X = F(Fx^), Z = F(Fy^); then sigma F(Fz^)
       fwd(FWD{});
-      each([&](int q, int r, int l, int n, bool ok) {
-        if (kp.kept(n, N)) ybuf[n * TM + l] = X[q][r];
-      });
+      keep_first();
       fwd(FWD{});
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          oc[(long long)j * ols + l] = cscale(ybuf[n * TM + l] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+          oc[(long long)j * ols + l] = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
         }
       });
       cx<T>* o2 = out + (long long)(5 + c) * a.out_fs + c0;
@@ -397,9 +462,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       // CM_FWDINV_DIV / CM_FWDINV_COPY: forward, combine on kept modes, inverse
       fwd(FWD{});
       if constexpr (MODE == CM_FWDINV_DIV) {
-        each([&](int q, int r, int l, int n, bool ok) {
-          if (kp.kept(n, N)) ybuf[n * TM + l] = X[q][r];
-        });
+        keep_first();
         fwd(FWD{});
       }
       each([&](int q, int r, int l, int n, bool ok) {
@@ -407,7 +470,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         if (kp.kept(n, N)) {
           const int j = kp.compact(n, N);
           if constexpr (MODE == CM_FWDINV_DIV) {
-            v = cscale(ybuf[n * TM + l] + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+            v = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
           } else {
             v = cscale(X[q][r], a.sig[j]);
             if (a.ikflag) v = mul_ik(v, a.kk[j]);
@@ -479,11 +542,11 @@ __device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, const
     if (l < nfl) {
       const cx<T>* s = src + (long long)l * a.in_fs;
       if (n < a.mx) {
-        const cx<T> A = s[n], B = s[a.mxp + n];
+        const cx<T> A = spec_ld(s, n), B = spec_ld(s + a.mxp, n);
         z = (n == 0 || 2 * n == NX) ? mk(A.x, B.x) : mk(A.x - B.y, A.y + B.x);
       } else if (n > NX - a.mx) {
         const int k = NX - n;
-        const cx<T> A = s[k], B = s[a.mxp + k];
+        const cx<T> A = spec_ld(s, k), B = spec_ld(s + a.mxp, k);
         z = mk(A.x + B.y, B.x - A.y);
       }
     }
@@ -636,10 +699,8 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
     for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * xstride(NC) + c) + line];
     if constexpr (PH_XMID_PREFETCH) rk_point<T, NC, NC == 4>(a, v, o, dtd, u, pre_un[pi], pre_acc[pi]);
     else rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
-    if (a.stage == 4) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
-    }
+    // stage 4: the new state goes on only as its x transform; the post-step
+    // filter that always follows writes the state itself
 #pragma unroll
     for (int c = 0; c < NC; ++c) q[pi][c] = u[c];
   }

@@ -225,7 +225,10 @@ struct Plan3D : PlanBase {
   int nx, ny, nz;
   double lx, ly, lz, gamma;
   AxisTables ax, ay, az;
-  DevAxis<T> dx, dy, dz;
+  using CE = typename ColElem<T>::type;  // column-pass element
+  static constexpr int PW = ColElem<T>::PW;
+  DevAxis<T> dx;
+  DevAxis<CE> dy, dz;
   int mx, mxp;
   long long ncell;
   // workspace layout (bytes)
@@ -315,16 +318,16 @@ struct Plan3D : PlanBase {
   }
   // y forward: A -> B
   void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
-    ColArgs<T> a{};
-    a.in = A(ws);
-    a.in_fs = a_fs();
-    a.in_bs = (long long)ny * mxp;
-    a.in_ls = mxp;
-    a.out = B(ws);
-    a.out_fs = b_fs();
-    a.out_bs = (long long)ay.nr * mxp;
-    a.out_ls = mxp;
-    a.ncol = mx;
+    ColArgs<CE> a{};
+    a.in = reinterpret_cast<cx<CE>*>(A(ws));
+    a.in_fs = (a_fs()) / PW;
+    a.in_bs = ((long long)ny * mxp) / PW;
+    a.in_ls = (mxp) / PW;
+    a.out = reinterpret_cast<cx<CE>*>(B(ws));
+    a.out_fs = (b_fs()) / PW;
+    a.out_bs = ((long long)ay.nr * mxp) / PW;
+    a.out_ls = (mxp) / PW;
+    a.ncol = (mx + PW - 1) / PW;
     a.nc = nc;
     a.keep = ay.keep();
     a.ikflag = ikflag;
@@ -332,22 +335,22 @@ struct Plan3D : PlanBase {
     a.sig = dy.sig;
     a.kk = dy.k;
     KScope ks(div ? "yfwd_div" : "yfwd_copy", st);
-    if (div) launch_col<T, CM_FWD_DIV3>(ny, nz, a, st);
-    else launch_col<T, CM_FWD_COPY>(ny, nz, a, st);
+    if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
+    else launch_col<CE, CM_FWD_COPY>(ny, nz, a, st);
     check_launch("k_col(y fwd)");
   }
   // z forward + combine + inverse, in place in B
   void zpass(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
-    ColArgs<T> a{};
-    a.in = B(ws);
-    a.in_fs = b_fs();
-    a.in_bs = mxp;
-    a.in_ls = (long long)ay.nr * mxp;
-    a.out = B(ws);
+    ColArgs<CE> a{};
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = (mxp) / PW;
+    a.in_ls = ((long long)ay.nr * mxp) / PW;
+    a.out = reinterpret_cast<cx<CE>*>(B(ws));
     a.out_fs = a.in_fs;
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
-    a.ncol = mx;
+    a.ncol = (mx + PW - 1) / PW;
     a.nc = nc;
     a.keep = az.keep();
     a.ikflag = ikflag;
@@ -355,22 +358,22 @@ struct Plan3D : PlanBase {
     a.sig = dz.sig;
     a.kk = dz.k;
     KScope ks(div ? "z_div" : "z_copy", st);
-    if (div) launch_col<T, CM_FWDINV_DIV>(nz, ay.nr, a, st);
-    else launch_col<T, CM_FWDINV_COPY>(nz, ay.nr, a, st);
+    if (div) launch_col<CE, CM_FWDINV_DIV>(nz, ay.nr, a, st);
+    else launch_col<CE, CM_FWDINV_COPY>(nz, ay.nr, a, st);
     check_launch("k_col(z)");
   }
   // y inverse: B (compact) -> A (full)
   void yinv(int nc, void* ws, hipStream_t st) const {
-    ColArgs<T> a{};
-    a.in = B(ws);
-    a.in_fs = b_fs();
-    a.in_bs = (long long)ay.nr * mxp;
-    a.in_ls = mxp;
-    a.out = A(ws);
-    a.out_fs = a_fs();
-    a.out_bs = (long long)ny * mxp;
-    a.out_ls = mxp;
-    a.ncol = mx;
+    ColArgs<CE> a{};
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = ((long long)ay.nr * mxp) / PW;
+    a.in_ls = (mxp) / PW;
+    a.out = reinterpret_cast<cx<CE>*>(A(ws));
+    a.out_fs = (a_fs()) / PW;
+    a.out_bs = ((long long)ny * mxp) / PW;
+    a.out_ls = (mxp) / PW;
+    a.ncol = (mx + PW - 1) / PW;
     a.nc = nc;
     a.keep = ay.keep();
     a.ikflag = 0;
@@ -378,7 +381,7 @@ struct Plan3D : PlanBase {
     a.sig = dy.sig;
     a.kk = dy.k;
     KScope ks("yinv", st);
-    launch_col<T, CM_INV>(ny, nz, a, st);
+    launch_col<CE, CM_INV>(ny, nz, a, st);
     check_launch("k_col(y inv)");
   }
   XInvArgs<T> xinv_args(void* ws) const {
@@ -489,7 +492,10 @@ struct Plan2D : PlanBase {
   int nx, ny, batch;
   double lx, ly, gamma;
   AxisTables ax, ay;
-  DevAxis<T> dx, dy;
+  using CE = typename ColElem<T>::type;  // column-pass element
+  static constexpr int PW = ColElem<T>::PW;
+  DevAxis<T> dx;
+  DevAxis<CE> dy;
   int mx, mxp;
   long long ncell;
   size_t offA, offUs, offPart, wsBytes;
@@ -568,25 +574,25 @@ struct Plan2D : PlanBase {
     check_launch("k_xfwd(2d)");
   }
   void ypass(bool div, int ikflag, int nb, void* ws, hipStream_t st, const int* active = nullptr) const {
-    ColArgs<T> a{};
+    ColArgs<CE> a{};
     a.active = active;
-    a.in = A(ws);
-    a.in_fs = (long long)ny * mxp;
-    a.in_bs = 8ll * ny * mxp;
-    a.in_ls = mxp;
-    a.out = A(ws);
+    a.in = reinterpret_cast<cx<CE>*>(A(ws));
+    a.in_fs = ((long long)ny * mxp) / PW;
+    a.in_bs = (8ll * ny * mxp) / PW;
+    a.in_ls = (mxp) / PW;
+    a.out = reinterpret_cast<cx<CE>*>(A(ws));
     a.out_fs = a.in_fs;
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
-    a.ncol = mx;
+    a.ncol = (mx + PW - 1) / PW;
     a.nc = 4;
     a.keep = ay.keep();
     a.ikflag = ikflag;
     a.tw = dy.tw;
     a.sig = dy.sig;
     a.kk = dy.k;
-    if (div) launch_col<T, CM_FWDINV_DIV>(ny, nb, a, st);
-    else launch_col<T, CM_FWDINV_COPY>(ny, nb, a, st);
+    if (div) launch_col<CE, CM_FWDINV_DIV>(ny, nb, a, st);
+    else launch_col<CE, CM_FWDINV_COPY>(ny, nb, a, st);
     check_launch("k_col(2d y)");
   }
   XInvArgs<T> xinv_args(void* ws) const {
