@@ -149,11 +149,23 @@ struct LastRegs {
 };
 
 // load the R inputs of butterfly b of pass (R, Ns) and apply the twiddles
-template <typename T, int N, int L, int DIR, int R, int NS, int LS = L>
+// LDS slot of (position, line) for line stride LS; PADB > 0 inserts one
+// empty position after every 2^PADB positions (breaks power-of-two strides
+// across LDS banks).
+template <int LS, int PADB>
+__device__ __forceinline__ int lds_slot(int pos, int l) {
+  return (PADB > 0 ? pos + (pos >> PADB) : pos) * LS + l;
+}
+template <int N, int LS, int PADB>
+struct LdsSize {
+  static constexpr int value = (PADB > 0 ? N + (N >> PADB) : N) * LS;
+};
+
+template <typename T, int N, int L, int DIR, int R, int NS, int LS = L, int PADB = 0>
 __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int b, cx<T>* v) {
   const int l = b % L, j = b / L;
 #pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = buf[(j + r * (N / R)) * LS + l];
+  for (int r = 0; r < R; ++r) v[r] = buf[lds_slot<LS, PADB>(j + r * (N / R), l)];
   if constexpr (NS > 1) {
     const int k = j % NS;
 #pragma unroll
@@ -169,7 +181,7 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
 // one Stockham pass, LDS -> LDS (in place; ends with a barrier).  Lines are
 // interleaved with stride LS >= L (padding keeps 16-byte accesses free of
 // bank conflicts when L is not a multiple of 8).
-template <typename T, int N, int L, int DIR, int NT, int P, int LS = L>
+template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0>
 __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
   constexpr int R = Plan<N>::radix(P);
   constexpr int NS = PassNs<N, P>::value;
@@ -179,7 +191,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS>(buf, tw, b, v[q]);
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS, PADB>(buf, tw, b, v[q]);
   }
   lds_barrier();
 #pragma unroll
@@ -190,26 +202,26 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
       const int k = j % NS;
       const int d = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[(d + r * NS) * LS + l] = v[q][r];
+      for (int r = 0; r < R; ++r) buf[lds_slot<LS, PADB>(d + r * NS, l)] = v[q][r];
     }
   }
   lds_barrier();
 }
 
-template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L>
+template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L, int PADB = 0>
 __device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
   if constexpr (P < PEND) {
-    pass_lds<T, N, L, DIR, NT, P, LS>(buf, tw, tid);
-    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS>(buf, tw, tid);
+    pass_lds<T, N, L, DIR, NT, P, LS, PADB>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB>(buf, tw, tid);
   }
 }
 
 // Full transform of the L lines in buf (line stride LS), result in buf
 // (natural order).  Caller must have synchronised after filling buf; ends
 // with a barrier.
-template <typename T, int N, int L, int DIR, int NT, int LS = L>
+template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0>
 __device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass, LS>(buf, tw, tid);
+  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass, LS, PADB>(buf, tw, tid);
 }
 
 // Full transform with the last pass left in registers v[q][r]; output r of

@@ -120,6 +120,14 @@ struct XFwdArgs {
 #define PH_XLS 0
 #endif
 constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : lines; }
+// optional position padding of the x kernels' LDS lines (see lds_slot)
+#ifndef PH_XPADB
+#define PH_XPADB 0
+#endif
+template <int LINES>
+__device__ __forceinline__ int xslot(int pos, int l) { return lds_slot<xstride(LINES), PH_XPADB>(pos, l); }
+template <int NX, int LINES>
+constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value; }
 
 // Twiddles of the x kernels: an LDS copy per block (PH_XTW_GLOBAL 0), or read
 // where they lie in global memory through the L1 cache (1), which takes the
@@ -202,12 +210,12 @@ __device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, 
               if (l == g + 1) v = v + p_;
             }
           }
-          bufr[2 * (x * xstride(LX) + l) + line] = v;
+          bufr[2 * xslot<LX>(x, l) + line] = v;
         }
       }
     }
     lds_barrier();
-    fft_lds<T, NX, LX, -1, NT, xstride(LX)>(buf, tw, tid);
+    fft_lds<T, NX, LX, -1, NT, xstride(LX), PH_XPADB>(buf, tw, tid);
     // split the two real spectra, keep mx modes, scale, write two rows
     for (int i = tid; i < LX * a.mxp; i += NT) {
       const int l = i / a.mxp, k = i - l * a.mxp;
@@ -215,8 +223,8 @@ __device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, 
       if (MODE == XF_COPY && l >= ncomp) continue;
       cx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
       if (k < a.mx) {
-        const cx<T> zk = buf[k * xstride(LX) + l];
-        const cx<T> zn = buf[((NX - k) & (NX - 1)) * xstride(LX) + l];
+        const cx<T> zk = buf[xslot<LX>(k, l)];
+        const cx<T> zn = buf[xslot<LX>((NX - k) & (NX - 1), l)];
         A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
         B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
         const T s = a.sig[k];
@@ -243,7 +251,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a
   constexpr int LX = S::LX, NC = S::NC;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xstride(LX) * NX];
+  __shared__ cx<T> buf[xbuf_size<NX, LX>()];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
@@ -550,10 +558,10 @@ __device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, const
         z = mk(A.x + B.y, B.x - A.y);
       }
     }
-    buf[n * xstride(NC) + l] = z;
+    buf[xslot<NC>(n, l)] = z;
   }
   lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC)>(buf, tw, tid);
+  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
 }
 
 // RK4 stage combine at one point (phrike/solver.py:1220-1230 3D, 502-513 2D)
@@ -598,7 +606,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xstride(NC) * NX];
+  __shared__ cx<T> buf[xbuf_size<NX, NC>()];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
@@ -617,7 +625,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
     const long long o = base + p;  // rows y0, y0+1 are contiguous
     T v[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * xstride(NC) + c) + line];
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * xslot<NC>(x, c) + line];
     if constexpr (MODE == XI_OUT) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
@@ -658,7 +666,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
   // inverse (NC lines), forward (FMODE lines) and the stage-4 copy (5 lines)
   constexpr int LB0 = xstride(NC) > xstride(XFwdShape<FMODE>::LX) ? xstride(NC) : xstride(XFwdShape<FMODE>::LX);
   constexpr int LB = LB0 > xstride(XFwdShape<XF_COPY>::LX) ? LB0 : xstride(XFwdShape<XF_COPY>::LX);
-  __shared__ cx<T> buf[LB * NX];
+  __shared__ cx<T> buf[LdsSize<NX, LB, PH_XPADB>::value];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
@@ -696,7 +704,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m
     const long long o = base + p;
     T v[NC], u[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * (x * xstride(NC) + c) + line];
+    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * xslot<NC>(x, c) + line];
     if constexpr (PH_XMID_PREFETCH) rk_point<T, NC, NC == 4>(a, v, o, dtd, u, pre_un[pi], pre_acc[pi]);
     else rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
     // stage 4: the new state goes on only as its x transform; the post-step

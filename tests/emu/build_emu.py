@@ -18,6 +18,8 @@ SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "hybrid.hip", "emu_stubs.
 def build(asan: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-I", HERE, "-Wno-unused-result"]
+    # extra defines for tuning variants of the kernels, e.g. EMU_DEFINES="-DPH_XPADB=4"
+    flags += os.environ.get("EMU_DEFINES", "").split()
     if asan:
         flags += ["-fsanitize=address", "-fno-omit-frame-pointer"]
     objs = []
