@@ -51,15 +51,17 @@ def byte_model(n: int):
 
 
 def kernel_symbol(tag: str, n: int, prec: str) -> str:
-    """The HIP kernel (as rocprofv3 names it) that a timing tag launches."""
+    """The HIP kernel (as rocprofv3 names it) that a timing tag launches
+    (x kernels instantiated for the 2/3 rule: kept-mode flag `true`)."""
     T = "double" if prec == "double" else "float"
-    return {"xfwd_flux": f"k_xfwd<{T}, {n}, 0>", "xfwd_copy": f"k_xfwd<{T}, {n}, 2>",
-            "yfwd_div": f"k_col<{T}, {n}, 0>", "yfwd_copy": f"k_col<{T}, {n}, 1>",
-            "z_div": f"k_col<{T}, {n}, 2>", "z_copy": f"k_col<{T}, {n}, 3>",
-            "yinv": f"k_col<{T}, {n}, 4>", "xinv_out": f"k_xinv<{T}, {n}, 5, 0>",
+    C = "double" if prec == "double" else "ph::f2"
+    return {"xfwd_flux": f"k_xfwd<{T}, {n}, true, 0>", "xfwd_copy": f"k_xfwd<{T}, {n}, true, 2>",
+            "yfwd_div": f"k_col<{C}, {n}, 0>", "yfwd_copy": f"k_col<{C}, {n}, 1>",
+            "z_div": f"k_col<{C}, {n}, 2>", "z_copy": f"k_col<{C}, {n}, 3>",
+            "yinv": f"k_col<{C}, {n}, 4>", "xinv_out": f"k_xinv<{T}, {n}, true, 5, 0>",
             # one kernel for every stage boundary (stage 4 branches at run time)
-            "xmid1": f"k_xmid<{T}, {n}, 5, 0>", "xmid23": f"k_xmid<{T}, {n}, 5, 0>",
-            "xmid_last": f"k_xmid<{T}, {n}, 5, 0>"}.get(tag, tag)
+            "xmid1": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid23": f"k_xmid<{T}, {n}, true, 5, 0>",
+            "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>"}.get(tag, tag)
 
 
 # launches of each timing tag in one RK4 step of the fused 3D pipeline

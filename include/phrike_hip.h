@@ -69,6 +69,15 @@ int ph_plan3d_create(ph_plan3d_t* plan, int nx, int ny, int nz, double lx, doubl
 int ph_plan3d_destroy(ph_plan3d_t plan);
 int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes);
 
+/* Reference pressure p_ref taken out of the pressure in the diagonal momentum
+ * fluxes: F = m u + (p - p_ref) instead of m u + p (phrike/equations.py:
+ * 167-196, 119-134).  The spectral derivative of a constant is zero, so the
+ * operator is unchanged; in FP32 the fluxes then round on the scale of the
+ * pressure variation rather than of its mean.  Default 0, which is the
+ * reference's expression.  No reference counterpart: a numerics setting of
+ * this backend for its FP32 plans; read at each launch. */
+int ph_plan3d_set_pressure_offset(ph_plan3d_t plan, double p_ref);
+
 /* rhs = -(dx Fx + dy Fy + dz Fz) from U (5 fields).
  * Replaces _compute_rhs_3d (phrike/solver.py:1177-1184) with
  * EulerEquations3D.flux (phrike/equations.py:582-607) and Grid3D.dx1/dy1/dz1
@@ -113,6 +122,8 @@ int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, in
                      double gamma, int batch);
 int ph_plan2d_destroy(ph_plan2d_t plan);
 int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes);
+/* As ph_plan3d_set_pressure_offset, for every run of the batch. */
+int ph_plan2d_set_pressure_offset(ph_plan2d_t plan, double p_ref);
 
 /* rhs = -(dx Fx + dy Fy) for the first nbatch runs.  Replaces
  * _compute_rhs_2d (phrike/solver.py:428-443). */

@@ -26,6 +26,7 @@ _SIGNATURES = {
                                  c_int, c_int, c_int, c_int, c_double, c_double]),
     "ph_plan3d_destroy": (c_int, [c_void_p]),
     "ph_plan3d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_plan3d_set_pressure_offset": (c_int, [c_void_p, c_double]),
     "ph_rhs3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_deriv3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
@@ -35,6 +36,7 @@ _SIGNATURES = {
                                  c_int, c_int, c_double, c_double, c_int]),
     "ph_plan2d_destroy": (c_int, [c_void_p]),
     "ph_plan2d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_plan2d_set_pressure_offset": (c_int, [c_void_p, c_double]),
     "ph_rhs2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_deriv2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,

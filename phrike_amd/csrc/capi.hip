@@ -1,5 +1,6 @@
 // C ABI (include/phrike_hip.h): argument checks, error strings, dispatch on
 // the plan's element type.
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -128,6 +129,13 @@ int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes) {
   });
 }
 
+int ph_plan3d_set_pressure_offset(ph_plan3d_t plan, double p_ref) {
+  return guard([&] {
+    need(std::isfinite(p_ref), "pressure offset must be finite");
+    with3(plan, [&](auto& p) { p.pref = p_ref; });
+  });
+}
+
 int ph_rhs3d(ph_plan3d_t plan, const void* U, void* rhs, void* ws, void* stream) {
   return guard([&] {
     need(U && rhs && ws, "null buffer");
@@ -210,6 +218,13 @@ int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes) {
   return guard([&] {
     need(bytes != nullptr, "null output");
     with2(plan, [&](auto& p) { *bytes = p.wsBytes; });
+  });
+}
+
+int ph_plan2d_set_pressure_offset(ph_plan2d_t plan, double p_ref) {
+  return guard([&] {
+    need(std::isfinite(p_ref), "pressure offset must be finite");
+    with2(plan, [&](auto& p) { p.pref = p_ref; });
   });
 }
 

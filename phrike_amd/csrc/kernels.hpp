@@ -53,34 +53,6 @@ struct ColElem<float> {
 };
 #endif
 
-// spectral row element k (x kernels), in the layout of ColElem
-template <typename T>
-__device__ __forceinline__ cx<T> spec_ld(const cx<T>* row, int k) {
-  if constexpr (ColElem<T>::PW == 2) {
-    const T* f = reinterpret_cast<const T*>(row) + (k >> 1) * 4 + (k & 1);
-    return mk(f[0], f[2]);
-  } else {
-    return row[k];
-  }
-}
-template <typename T>
-__device__ __forceinline__ void spec_st(cx<T>* row, int k, cx<T> v) {
-  if constexpr (ColElem<T>::PW == 2) {
-    T* f = reinterpret_cast<T*>(row) + (k >> 1) * 4 + (k & 1);
-    f[0] = v.x;
-    f[2] = v.y;
-  } else {
-    row[k] = v;
-  }
-}
-
-// Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
-// of the block's 2*NX points, which bounds the per-thread flux and FFT state.
-template <typename T, int NX>
-struct XThreads {
-  static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, 512) : clampi(NX / 2, 64, 256);
-};
-
 // ----------------------------------------------------------------------------
 // retained-mode bookkeeping along one axis: positions n <= kp and
 // n >= N - kn are kept; compact index j = n (n <= kp) or n - (N - nr).
@@ -93,24 +65,25 @@ struct Keep {
 };
 
 // ----------------------------------------------------------------------------
-// x forward
+// x kernels: forward (fluxes or copies -> kept x modes), inverse (kept x modes
+// -> physical rows, optionally with the RK4 stage combine) and the fused
+// stage boundary k_xmid.
+//
+// A block owns the two rows y0, y0+1 of one slab; their transforms along x run
+// as one complex transform per field of z = row0 + i*row1.  Thread t owns the
+// x positions t + j*NT (j < XJ) of both rows, so one complex LDS access moves
+// the value of a field at one x in both rows.  The kept-mode count MX
+// (NX/3 + 1 with the 2/3 rule, NX/2 + 1 without) is a compile-time constant;
+// spectral rows move in 16-byte chunks: one FP64 complex, or one pair-planar
+// FP32 pair (re(2m) re(2m+1) im(2m) im(2m+1)).  Only the chunks that hold
+// kept modes are read or written (the column passes skip the row padding).
 // ----------------------------------------------------------------------------
-enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2 };
 
-template <typename T>
-struct XFwdArgs {
-  const T* in;
-  long long in_fs, in_ss;  // field / slab strides (elements); rows of nx
-  cx<T>* out;
-  long long out_fs, out_ss;  // rows of mxp complex
-  int nf;                    // XF_COPY: number of fields (<= 5)
-  int mx, mxp;
-  unsigned ikmask;  // bit f: multiply field f by i*kx
-  const cx<T>* tw;
-  const T* sig;  // sigma_x(m) * scale, m < mx
-  const T* kk;   // kx(m), zero at the Nyquist mode
-  T gamma;
-  const int* active;  // per-slab flag (batched 2D) or null
+// Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
+// of the block's 2*NX points, which bounds the per-thread flux and FFT state.
+template <typename T, int NX>
+struct XThreads {
+  static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, 512) : clampi(NX / 2, 64, 256);
 };
 
 // LDS line stride (complex values per x position) of the x kernels: by
@@ -130,8 +103,7 @@ template <int NX, int LINES>
 constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value; }
 
 // Twiddles of the x kernels: an LDS copy per block (PH_XTW_GLOBAL 0), or read
-// where they lie in global memory through the L1 cache (1), which takes the
-// table out of LDS and lets one more block share a CU.
+// where they lie in global memory through the L1 cache (1).
 #ifndef PH_XTW_GLOBAL
 #define PH_XTW_GLOBAL 0
 #endif
@@ -143,136 +115,311 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #define PH_XTW_FILL(src, NX, NT, tid) \
   for (int i_ = (tid); i_ < (NX); i_ += (NT)) tw[i_] = (src)[i_]
 #endif
+// k_xmid loads the RK combine's inputs before its inverse transform (1: U and
+// the accumulator, 2: U only, 0: none) so their latency overlaps it
 #ifndef PH_XMID_PREFETCH
 #define PH_XMID_PREFETCH 1
 #endif
+// minimum waves per SIMD requested for the x kernels (1: compiler's choice)
+#ifndef PH_X_WAVES_PER_EU
+#define PH_X_WAVES_PER_EU 1
+#endif
+
+template <typename T, int NX, bool DA>
+struct XGeo {
+  static constexpr int NT = XThreads<T, NX>::value;
+  static constexpr int MX = DA ? NX / 3 + 1 : NX / 2 + 1;   // kept x modes
+  static constexpr int K = ColElem<T>::PW;                   // modes per chunk
+  static constexpr int MC = (MX + K - 1) / K;                // chunks holding kept modes
+  static constexpr bool XALL = NX >= NT;                     // every thread owns positions
+  static constexpr int XJ = XALL ? NX / NT : 1;              // x positions per thread
+  static constexpr int ZB = NX - 2 * MX + 1 > 0 ? NX - 2 * MX + 1 : 0;  // zero band of an inverse input
+};
+
+// the 16-byte unit of a spectral row
+struct alignas(16) pair4f {
+  float a, b, c, d;
+};
+template <typename T, int K>
+struct ChunkT { using type = cx<T>; };
+template <>
+struct ChunkT<float, 2> { using type = pair4f; };
+template <typename T>
+__device__ __forceinline__ cx<T> ch_get(const cx<T>& c, int) { return c; }
+__device__ __forceinline__ cx<float> ch_get(const pair4f& c, int i) { return i == 0 ? mk(c.a, c.c) : mk(c.b, c.d); }
+template <typename T>
+__device__ __forceinline__ void ch_set(cx<T>& c, int, cx<T> v) { c = v; }
+__device__ __forceinline__ void ch_set(pair4f& c, int i, cx<float> v) {
+  if (i == 0) {
+    c.a = v.x;
+    c.c = v.y;
+  } else {
+    c.b = v.x;
+    c.d = v.y;
+  }
+}
+
+// quotient of the flux primitives: IEEE division in FP64 (the reference's
+// operation); FP32 multiplies by the hardware reciprocal (1 ulp)
+__device__ __forceinline__ double xrcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float xrcp(float v) { return __builtin_amdgcn_rcpf(v); }
+template <typename T>
+__device__ __forceinline__ T xquot(T a, T b, T rb) {
+  if constexpr (sizeof(T) == 8) return a / b;
+  else return a * rb;
+}
+
+// Kept half spectra of rows y0, y0+1 of NCI fields, loaded into registers in
+// 16-byte chunks; unsplit() writes z = row0 + i*row1 in natural order to LDS
+// (line l = field l), with zeros in the band MX..NX-MX.
+template <typename T, int NX, bool DA, int NCI>
+struct XSpec {
+  using G = XGeo<T, NX, DA>;
+  using C = typename ChunkT<T, G::K>::type;
+  static constexpr int NTASK = NCI * G::MC;
+  static constexpr int PER = (NTASK + G::NT - 1) / G::NT;
+  C a[PER], b[PER];
+
+  // src: row y0 of field 0; fs and pitch in complex values
+  __device__ __forceinline__ void load(const cx<T>* src, long long fs, int pitch, int nfl, int tid) {
+    const C* s = reinterpret_cast<const C*>(src);
+    const long long fsc = fs / G::K;
+    const int pc = pitch / G::K;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = tid + q * G::NT;
+      a[q] = C{};
+      b[q] = C{};
+      if (NTASK % G::NT == 0 || t < NTASK) {
+        const int l = t / G::MC, m = t - l * G::MC;
+        if (l < nfl) {
+          const C* p = s + l * fsc + m;
+          a[q] = p[0];
+          b[q] = p[pc];
+        }
+      }
+    }
+  }
+
+  template <int LS, int PADB>
+  __device__ __forceinline__ void unsplit(cx<T>* buf, int tid) const {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = tid + q * G::NT;
+      if (NTASK % G::NT == 0 || t < NTASK) {
+        const int l = t / G::MC, m = t - l * G::MC;
+#pragma unroll
+        for (int i = 0; i < G::K; ++i) {
+          const int k = m * G::K + i;
+          if (k < G::MX) {
+            const cx<T> A = ch_get(a[q], i), B = ch_get(b[q], i);
+            if (k == 0 || 2 * k == NX) {
+              buf[lds_slot<LS, PADB>(k, l)] = mk(A.x, B.x);
+            } else {
+              buf[lds_slot<LS, PADB>(k, l)] = mk(A.x - B.y, A.y + B.x);
+              buf[lds_slot<LS, PADB>(NX - k, l)] = mk(A.x + B.y, B.x - A.y);
+            }
+          }
+        }
+      }
+    }
+    if constexpr (G::ZB > 0) {
+      constexpr int NZ = NCI * G::ZB;
+      for (int t = tid; t < NZ; t += G::NT) {
+        const int l = t / G::ZB, n = G::MX + (t - l * G::ZB);
+        buf[lds_slot<LS, PADB>(n, l)] = mk(T(0), T(0));
+      }
+    }
+  }
+};
+
+// Split the transforms of z = row0 + i*row1 (LX lines in buf, natural order)
+// into the two rows' half spectra, keep MX modes, scale by sig (and i*k),
+// store lines l < nl to dst (row y0 of the group's first field).
+template <typename T, int NX, bool DA, int LX, int LS, int PADB>
+__device__ __forceinline__ void x_split_store(const cx<T>* buf, cx<T>* dst, long long fs, int pitch, int nl,
+                                              const T* sig, const T* kk, bool ik, int tid) {
+  using G = XGeo<T, NX, DA>;
+  using C = typename ChunkT<T, G::K>::type;
+  constexpr int NTASK = LX * G::MC;
+  C* d = reinterpret_cast<C*>(dst);
+  const long long fsc = fs / G::K;
+  const int pc = pitch / G::K;
+  for (int t = tid; t < NTASK; t += G::NT) {
+    const int l = t / G::MC, m = t - l * G::MC;
+    if (l < nl) {
+      C ca{}, cb{};
+#pragma unroll
+      for (int i = 0; i < G::K; ++i) {
+        const int k = m * G::K + i;
+        if (k < G::MX) {
+          const cx<T> zk = buf[lds_slot<LS, PADB>(k, l)];
+          const cx<T> zn = buf[lds_slot<LS, PADB>((NX - k) & (NX - 1), l)];
+          cx<T> A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
+          cx<T> B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
+          const T s = sig[k];
+          A = cscale(A, s);
+          B = cscale(B, s);
+          if (ik) {
+            const T kx = kk[k];
+            A = mul_ik(A, kx);
+            B = mul_ik(B, kx);
+          }
+          ch_set(ca, i, A);
+          ch_set(cb, i, B);
+        }
+      }
+      C* p = d + l * fsc + m;
+      p[0] = ca;
+      p[pc] = cb;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// x forward
+// ----------------------------------------------------------------------------
+enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2 };
+
+template <typename T>
+struct XFwdArgs {
+  const T* in;
+  long long in_fs, in_ss;  // field / slab strides (elements); rows of nx
+  cx<T>* out;
+  long long out_fs, out_ss;  // rows of mxp complex
+  int nf;                    // XF_COPY: number of fields (<= 5)
+  int mx, mxp;
+  unsigned ikmask;  // bit f: multiply field f by i*kx (uniform over a group of lines)
+  const cx<T>* tw;
+  const T* sig;  // sigma_x(m) * scale, m < mx
+  const T* kk;   // kx(m), zero at the Nyquist mode
+  T gamma;
+  T pref;        // constant taken out of the pressure in the momentum fluxes
+  const int* active;  // per-slab flag (batched 2D) or null
+};
 
 template <int MODE> struct XFwdShape;
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 
-// Fluxes (or copies) of the conserved values q held in registers -> forward
-// x transform of two rows at once -> kept modes to a.out.  Shared by k_xfwd
-// and the fused k_xmid.
-template <typename T, int NX, int MODE, int NT, int PPT>
-__device__ __forceinline__ void xfwd_from_regs(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
-                                               int tid, int slab, int y0,
-                                               const T (&q)[PPT][XFwdShape<MODE>::NC]) {
+// Fluxes (or copies) of the conserved values q (both rows of the thread's x
+// positions) -> forward x transform of two rows at once -> kept modes to
+// a.out.  Shared by k_xfwd and the fused k_xmid.  A constant pref is taken
+// out of the pressure in the diagonal momentum fluxes (its derivative is zero;
+// with pref = 0 the fluxes are the reference's expressions).
+template <typename T, int NX, bool DA, int MODE>
+__device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
+                                          const T* sig, const T* kk, int tid, int slab, int y0,
+                                          const T (&q)[XGeo<T, NX, DA>::XJ][2][XFwdShape<MODE>::NC]) {
+  using G = XGeo<T, NX, DA>;
   using S = XFwdShape<MODE>;
-  constexpr int LX = S::LX, NC = S::NC;
-  T* bufr = reinterpret_cast<T*>(buf);
+  constexpr int LX = S::LX, NC = S::NC, NT = G::NT, XJ = G::XJ;
   // primitive recovery (phrike/equations.py:104-117, 150-164)
-  T pr[PPT][6];
+  T pr[XJ][2][6];
   if constexpr (MODE != XF_COPY) {
 #pragma unroll
-    for (int pi = 0; pi < PPT; ++pi) {
-      const T rho = q[pi][0];
-      if constexpr (MODE == XF_FLUX3) {
-        const T ux = q[pi][1] / rho, uy = q[pi][2] / rho, uz = q[pi][3] / rho;
-        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-        const T p = (a.gamma - T(1)) * (q[pi][4] - kin);
-        pr[pi][0] = rho; pr[pi][1] = ux; pr[pi][2] = uy; pr[pi][3] = uz; pr[pi][4] = p;
-        pr[pi][5] = q[pi][4] + p;  // E + p
-      } else {
-        const T ux = q[pi][1] / rho, uy = q[pi][2] / rho;
-        const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
-        const T p = (a.gamma - T(1)) * (q[pi][3] - kin);
-        pr[pi][0] = rho; pr[pi][1] = ux; pr[pi][2] = uy; pr[pi][3] = T(0); pr[pi][4] = p;
-        pr[pi][5] = q[pi][3] + p;
+    for (int j = 0; j < XJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const T* u = q[j][r];
+        const T rho = u[0];
+        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
+        if constexpr (MODE == XF_FLUX3) {
+          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
+          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+          const T p = (a.gamma - T(1)) * (u[4] - kin);
+          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = uz;
+          pr[j][r][4] = p - a.pref;
+          pr[j][r][5] = u[4] + p;  // E + p
+        } else {
+          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri);
+          const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+          const T p = (a.gamma - T(1)) * (u[3] - kin);
+          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = T(0);
+          pr[j][r][4] = p - a.pref;
+          pr[j][r][5] = u[3] + p;
+        }
       }
-    }
   }
-
   cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y0 * a.mxp;
+  const int nl = MODE == XF_COPY ? ncomp : LX;
 #pragma unroll 1
   for (int g = 0; g < S::NG; ++g) {
-    // fill LX complex lines: line l holds field g*LX+l of row y0 (re) and y0+1 (im)
 #pragma unroll
-    for (int pi = 0; pi < PPT; ++pi) {
-      const int p = tid + pi * NT;
-      if (p < 2 * NX) {
-        const int line = p / NX, x = p - line * NX;
+    for (int j = 0; j < XJ; ++j) {
+      const int x = tid + j * NT;
+      if (G::XALL || x < NX) {
 #pragma unroll
         for (int l = 0; l < LX; ++l) {
-          T v;
-          if constexpr (MODE == XF_COPY) {
-            v = q[pi][l];
-          } else {
-            // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
-            const T rho = pr[pi][0], p_ = pr[pi][4];
-            const T ud = pr[pi][1 + g];
-            const T md = rho * ud;
-            if (l == 0) v = md;
-            else if (l == NC - 1) v = pr[pi][5] * ud;
-            else {
-              v = md * pr[pi][l];
-              if (l == g + 1) v = v + p_;
+          T v[2];
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            if constexpr (MODE == XF_COPY) {
+              v[r] = q[j][r][l];
+            } else {
+              // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
+              const T* w = pr[j][r];
+              const T ud = w[1 + g];
+              const T md = w[0] * ud;
+              if (l == 0) v[r] = md;
+              else if (l == NC - 1) v[r] = w[5] * ud;
+              else {
+                v[r] = md * w[l];
+                if (l == g + 1) v[r] = v[r] + w[4];
+              }
             }
           }
-          bufr[2 * xslot<LX>(x, l) + line] = v;
+          buf[xslot<LX>(x, l)] = mk(v[0], v[1]);
         }
       }
     }
     lds_barrier();
     fft_lds<T, NX, LX, -1, NT, xstride(LX), PH_XPADB>(buf, tw, tid);
-    // split the two real spectra, keep mx modes, scale, write two rows
-    for (int i = tid; i < LX * a.mxp; i += NT) {
-      const int l = i / a.mxp, k = i - l * a.mxp;
-      const int f = g * LX + l;
-      if (MODE == XF_COPY && l >= ncomp) continue;
-      cx<T> A = mk(T(0), T(0)), B = mk(T(0), T(0));
-      if (k < a.mx) {
-        const cx<T> zk = buf[xslot<LX>(k, l)];
-        const cx<T> zn = buf[xslot<LX>((NX - k) & (NX - 1), l)];
-        A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
-        B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
-        const T s = a.sig[k];
-        A = cscale(A, s);
-        B = cscale(B, s);
-        if ((a.ikmask >> f) & 1u) {
-          const T kx = a.kk[k];
-          A = mul_ik(A, kx);
-          B = mul_ik(B, kx);
-        }
-      }
-      cx<T>* row = dst + (long long)f * a.out_fs;
-      spec_st(row, k, A);
-      spec_st(row + a.mxp, k, B);
-    }
+    x_split_store<T, NX, DA, LX, xstride(LX), PH_XPADB>(buf, dst + (long long)(g * LX) * a.out_fs, a.out_fs,
+                                                        a.mxp, nl, sig, kk, (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();
   }
 }
 
-template <typename T, int NX, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xfwd(XFwdArgs<T> a) {
-  constexpr int NT = XThreads<T, NX>::value;
+template <typename T, int NX, bool DA>
+__device__ __forceinline__ void x_tables(const T* sig, const T* kk, T* s_sig, T* s_k, int tid) {
+  using G = XGeo<T, NX, DA>;
+  for (int i = tid; i < G::MX; i += G::NT) {
+    s_sig[i] = sig[i];
+    s_k[i] = kk[i];
+  }
+}
+
+template <typename T, int NX, bool DA, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xfwd(XFwdArgs<T> a) {
+  using G = XGeo<T, NX, DA>;
   using S = XFwdShape<MODE>;
-  constexpr int LX = S::LX, NC = S::NC;
-  constexpr int PPT = (2 * NX + NT - 1) / NT;
+  constexpr int NT = G::NT, NC = S::NC, XJ = G::XJ;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xbuf_size<NX, LX>()];
+  __shared__ cx<T> buf[xbuf_size<NX, S::LX>()];
+  __shared__ T s_sig[G::MX], s_k[G::MX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
-
+  x_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
   const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
   const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
-  T q[PPT][NC];
+  T q[XJ][2][NC];
 #pragma unroll
-  for (int pi = 0; pi < PPT; ++pi) {
-    const int p = tid + pi * NT;
+  for (int j = 0; j < XJ; ++j) {
+    const int x = tid + j * NT;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) q[pi][c] = T(0);
-    if (p < 2 * NX) {
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
-        if (c < ncomp) q[pi][c] = src[(long long)c * a.in_fs + p];
-    }
+      for (int c = 0; c < NC; ++c) {
+        q[j][r][c] = T(0);
+        if ((G::XALL || x < NX) && c < ncomp) q[j][r][c] = src[(long long)c * a.in_fs + r * NX + x];
+      }
   }
-  xfwd_from_regs<T, NX, MODE, NT, PPT>(a, ncomp, tw, buf, tid, slab, y0, q);
+  x_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y0, q);
 }
 
 // ----------------------------------------------------------------------------
@@ -538,32 +685,6 @@ __device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
   u[3] = pc / (gamma - T(1)) + (T(0.5) * rc) * (ux * ux + uy * uy);
 }
 
-// Rebuild z = row0 + i*row1 from the kept half spectra of both rows of nfl
-// fields and inverse-transform along x; result in buf (natural order).
-template <typename T, int NX, int NC, int NT>
-__device__ __forceinline__ void xinv_to_lds(const XInvArgs<T>& a, int nfl, const cx<T>* tw, cx<T>* buf, int tid,
-                                            int slab, int y0) {
-  const cx<T>* src = a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp;
-  for (int i = tid; i < NC * NX; i += NT) {
-    const int l = i / NX, n = i - l * NX;
-    cx<T> z = mk(T(0), T(0));
-    if (l < nfl) {
-      const cx<T>* s = src + (long long)l * a.in_fs;
-      if (n < a.mx) {
-        const cx<T> A = spec_ld(s, n), B = spec_ld(s + a.mxp, n);
-        z = (n == 0 || 2 * n == NX) ? mk(A.x, B.x) : mk(A.x - B.y, A.y + B.x);
-      } else if (n > NX - a.mx) {
-        const int k = NX - n;
-        const cx<T> A = spec_ld(s, k), B = spec_ld(s + a.mxp, k);
-        z = mk(A.x + B.y, B.x - A.y);
-      }
-    }
-    buf[xslot<NC>(n, l)] = z;
-  }
-  lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
-}
-
 // RK4 stage combine at one point (phrike/solver.py:1220-1230 3D, 502-513 2D)
 // with k_s = -v: stages 1-3 update the accumulator a.out and return the next
 // stage state in u; stage 4 returns U + dt/6 (acc + k4) in u.  The 2D form
@@ -589,22 +710,21 @@ __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC],
   if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
 }
 
-template <typename T, int NC, bool CLAMP>
-__device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC], long long o, double dtd,
-                                         T (&u)[NC]) {
-  T un[NC], acc[NC];
+// the RK combine's inputs at one point: U at the start of the step and (from
+// stage 2 on) the accumulator
+template <typename T, int NC>
+__device__ __forceinline__ void rk_inputs(const XInvArgs<T>& a, long long o, T (&un)[NC], T (&acc)[NC]) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     un[c] = a.un[c * a.fs + o];
     acc[c] = a.stage > 1 ? a.out[c * a.fs + o] : T(0);
   }
-  rk_point<T, NC, CLAMP>(a, v, o, dtd, u, un, acc);
 }
 
-template <typename T, int NX, int NC, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a) {
-  constexpr int NT = XThreads<T, NX>::value;
-  constexpr int PPT = (2 * NX + NT - 1) / NT;
+template <typename T, int NX, bool DA, int NC, int MODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xinv(XInvArgs<T> a) {
+  using G = XGeo<T, NX, DA>;
+  constexpr int NT = G::NT, XJ = G::XJ;
   PH_XTW_DECL(T, NX);
   __shared__ cx<T> buf[xbuf_size<NX, NC>()];
   const int tid = threadIdx.x;
@@ -613,43 +733,53 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xinv(XInvArgs<T> a
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
   const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
-  xinv_to_lds<T, NX, NC, NT>(a, nfl, tw, buf, tid, slab, y0);
-  const T* bufr = reinterpret_cast<const T*>(buf);
+  XSpec<T, NX, DA, NC> sp;
+  sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
+  sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
+  lds_barrier();
+  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
   const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
 #pragma unroll
-  for (int pi = 0; pi < PPT; ++pi) {
-    const int p = tid + pi * NT;
-    if (p >= 2 * NX) continue;
-    const int line = p / NX, x = p - line * NX;
-    const long long o = base + p;  // rows y0, y0+1 are contiguous
-    T v[NC];
+  for (int j = 0; j < XJ; ++j) {
+    const int x = tid + j * NT;
+    if (!G::XALL && x >= NX) continue;
+    T v[2][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * xslot<NC>(x, c) + line];
-    if constexpr (MODE == XI_OUT) {
+    for (int c = 0; c < NC; ++c) {
+      const cx<T> z = buf[xslot<NC>(x, c)];
+      v[0][c] = z.x;
+      v[1][c] = z.y;
+    }
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
-        if (c < nfl) a.out[c * a.fs + o] = v[c];
-    } else if constexpr (MODE == XI_RHS) {
+    for (int r = 0; r < 2; ++r) {
+      const long long o = base + r * NX + x;
+      if constexpr (MODE == XI_OUT) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c)
-        if (c < nfl) a.out[c * a.fs + o] = -v[c];
-    } else {
-      T u[NC];
-      rk_point<T, NC, MODE == XI_STAGE2>(a, v, o, dtd, u);
-      T* dst = a.stage < 4 ? a.us : a.out;
+        for (int c = 0; c < NC; ++c)
+          if (c < nfl) a.out[c * a.fs + o] = v[r][c];
+      } else if constexpr (MODE == XI_RHS) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) dst[c * a.fs + o] = u[c];
+        for (int c = 0; c < NC; ++c)
+          if (c < nfl) a.out[c * a.fs + o] = -v[r][c];
+      } else {
+        T un[NC], acc[NC], u[NC];
+        rk_inputs<T, NC>(a, o, un, acc);
+        rk_point<T, NC, MODE == XI_STAGE2>(a, v[r], o, dtd, u, un, acc);
+        T* dst = a.stage < 4 ? a.us : a.out;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dst[c * a.fs + o] = u[c];
+      }
     }
   }
 }
 
 // Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
 // the fluxes of the stage s+1 state and their forward x transform, or (stage
-// 4) the new state written to a.out and its forward x transform for the
-// post-step filter.  The intermediate stage state never goes to HBM.  The
-// spectra are read from and written to the same rows of the same buffer; the
-// block loads its rows completely before it writes them.
+// 4) the new state's forward x transform for the post-step filter, which
+// always follows and writes the state itself.  The intermediate stage states
+// never go to HBM.  The spectra are read from and written to the same rows of
+// the same buffer; the block loads its rows completely before it writes them.
 template <typename T>
 struct XMidArgs {
   XInvArgs<T> inv;
@@ -657,71 +787,99 @@ struct XMidArgs {
   XFwdArgs<T> fwd_last;  // stage 4 (copy mode, 5 or 4 fields)
 };
 
-template <typename T, int NX, int NC, int FMODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value)) void k_xmid(XMidArgs<T> m) {
-  constexpr int NT = XThreads<T, NX>::value;
-  constexpr int PPT = (2 * NX + NT - 1) / NT;
+template <typename T, int NX, bool DA, int NC, int FMODE>
+__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xmid(XMidArgs<T> m) {
+  using G = XGeo<T, NX, DA>;
+  constexpr int NT = G::NT, XJ = G::XJ;
   static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
   PH_XTW_DECL(T, NX);
   // inverse (NC lines), forward (FMODE lines) and the stage-4 copy (5 lines)
   constexpr int LB0 = xstride(NC) > xstride(XFwdShape<FMODE>::LX) ? xstride(NC) : xstride(XFwdShape<FMODE>::LX);
   constexpr int LB = LB0 > xstride(XFwdShape<XF_COPY>::LX) ? LB0 : xstride(XFwdShape<XF_COPY>::LX);
   __shared__ cx<T> buf[LdsSize<NX, LB, PH_XPADB>::value];
+  __shared__ T s_sig[G::MX], s_k[G::MX];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
+  XSpec<T, NX, DA, NC> sp;
+  sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, NC, tid);
   PH_XTW_FILL(a.tw, NX, NT, tid);
+  x_tables<T, NX, DA>(m.fwd.sig, m.fwd.kk, s_sig, s_k, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
-  // with PH_XMID_PREFETCH the RK combine's inputs are loaded before the
-  // inverse transform so their latency overlaps it (more live registers)
-  T pre_un[PH_XMID_PREFETCH ? PPT : 1][NC], pre_acc[PH_XMID_PREFETCH ? PPT : 1][NC];
+  // the RK combine's inputs, loaded before the inverse transform so their
+  // latency overlaps it (PH_XMID_PREFETCH)
+  constexpr int PJ = PH_XMID_PREFETCH ? XJ : 1;
+  T pre_un[PJ][2][NC], pre_acc[PJ][2][NC];
+  if constexpr (PH_XMID_PREFETCH != 0) {
 #pragma unroll
-  for (int pi = 0; pi < (PH_XMID_PREFETCH ? PPT : 0); ++pi) {
-    const int p = tid + pi * NT;
+    for (int j = 0; j < XJ; ++j) {
+      const int x = tid + j * NT;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      pre_un[pi][c] = T(0);
-      pre_acc[pi][c] = T(0);
-      if (p < 2 * NX) {
-        pre_un[pi][c] = a.un[c * a.fs + base + p];
-        if (a.stage > 1) pre_acc[pi][c] = a.out[c * a.fs + base + p];
-      }
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          pre_un[j][r][c] = T(0);
+          pre_acc[j][r][c] = T(0);
+          if (G::XALL || x < NX) {
+            const long long o = base + r * NX + x;
+            pre_un[j][r][c] = a.un[c * a.fs + o];
+            if (PH_XMID_PREFETCH == 1 && a.stage > 1) pre_acc[j][r][c] = a.out[c * a.fs + o];
+          }
+        }
     }
   }
-  xinv_to_lds<T, NX, NC, NT>(a, NC, tw, buf, tid, slab, y0);
-  const T* bufr = reinterpret_cast<const T*>(buf);
+  sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
+  lds_barrier();
+  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
   const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
-  T q[PPT][5];
+  T q[XJ][2][5];
 #pragma unroll
-  for (int pi = 0; pi < PPT; ++pi) {
-    const int p = tid + pi * NT;
+  for (int j = 0; j < XJ; ++j) {
+    const int x = tid + j * NT;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) q[pi][c] = T(0);
-    if (p >= 2 * NX) continue;
-    const int line = p / NX, x = p - line * NX;
-    const long long o = base + p;
-    T v[NC], u[NC];
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = bufr[2 * xslot<NC>(x, c) + line];
-    if constexpr (PH_XMID_PREFETCH) rk_point<T, NC, NC == 4>(a, v, o, dtd, u, pre_un[pi], pre_acc[pi]);
-    else rk_point<T, NC, NC == 4>(a, v, o, dtd, u);
-    // stage 4: the new state goes on only as its x transform; the post-step
-    // filter that always follows writes the state itself
+      for (int c = 0; c < 5; ++c) q[j][r][c] = T(0);
+    if (!G::XALL && x >= NX) continue;
+    T v[2][NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) q[pi][c] = u[c];
+    for (int c = 0; c < NC; ++c) {
+      const cx<T> z = buf[xslot<NC>(x, c)];
+      v[0][c] = z.x;
+      v[1][c] = z.y;
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const long long o = base + r * NX + x;
+      T un[NC], acc[NC], u[NC];
+      if constexpr (PH_XMID_PREFETCH != 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          un[c] = pre_un[j][r][c];
+          acc[c] = PH_XMID_PREFETCH == 1 ? pre_acc[j][r][c] : (a.stage > 1 ? a.out[c * a.fs + o] : T(0));
+        }
+      } else {
+        rk_inputs<T, NC>(a, o, un, acc);
+      }
+      rk_point<T, NC, NC == 4>(a, v[r], o, dtd, u, un, acc);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) q[j][r][c] = u[c];
+    }
   }
   lds_barrier();  // buf is reused by the forward transform
   if (a.stage < 4) {
-    T qf[PPT][XFwdShape<FMODE>::NC];
+    T qf[XJ][2][XFwdShape<FMODE>::NC];
 #pragma unroll
-    for (int pi = 0; pi < PPT; ++pi)
+    for (int j = 0; j < XJ; ++j)
 #pragma unroll
-      for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[pi][c] = q[pi][c];
-    xfwd_from_regs<T, NX, FMODE, NT, PPT>(m.fwd, NC, tw, buf, tid, slab, y0, qf);
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
+    x_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y0, qf);
   } else {
-    xfwd_from_regs<T, NX, XF_COPY, NT, PPT>(m.fwd_last, NC, tw, buf, tid, slab, y0, q);
+    x_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y0, q);
   }
 }
 

@@ -120,10 +120,18 @@ struct DevAxis {
 
 inline bool supported_len(int n) { return n >= 8 && n <= 1024 && is_pow2(n); }
 
+// the x kernels are instantiated for both kept-mode counts: N/3 + 1 (2/3 rule)
+// and N/2 + 1
+template <typename T, int N, int MODE>
+void launch_xfwd_n(int mx, dim3 grid, const XFwdArgs<T>& a, hipStream_t st) {
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xfwd<T, N, true, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xfwd<T, N, false, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else throw Error(-2, "unsupported x mode count");
+}
 template <typename T, int MODE>
 void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t st) {
   dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xfwd<T, N_, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
+  PH_SWITCH_N(nx, (launch_xfwd_n<T, N_, MODE>(a.mx, grid, a, st)));
 }
 
 template <typename T, int MODE>
@@ -135,16 +143,28 @@ void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
   }));
 }
 
+template <typename T, int N, int NC, int MODE>
+void launch_xinv_n(int mx, dim3 grid, const XInvArgs<T>& a, hipStream_t st) {
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xinv<T, N, true, NC, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xinv<T, N, false, NC, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else throw Error(-2, "unsupported x mode count");
+}
 template <typename T, int NC, int MODE>
 void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t st) {
   dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xinv<T, N_, NC, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
+  PH_SWITCH_N(nx, (launch_xinv_n<T, N_, NC, MODE>(a.mx, grid, a, st)));
 }
 
+template <typename T, int N, int NC, int FMODE>
+void launch_xmid_n(int mx, dim3 grid, const XMidArgs<T>& a, hipStream_t st) {
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xmid<T, N, true, NC, FMODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xmid<T, N, false, NC, FMODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  else throw Error(-2, "unsupported x mode count");
+}
 template <typename T, int NC, int FMODE>
 void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
   dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, hipLaunchKernelGGL((k_xmid<T, N_, NC, FMODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
+  PH_SWITCH_N(nx, (launch_xmid_n<T, N_, NC, FMODE>(a.inv.mx, grid, a, st)));
 }
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
@@ -218,6 +238,9 @@ struct KScope {
 struct PlanBase {
   virtual ~PlanBase() = default;
   int dtype = 0;  // 0 = float64, 1 = float32
+  // constant taken out of the pressure in the momentum fluxes (0: the
+  // reference's expressions); set between calls, read at launch
+  double pref = 0.0;
 };
 
 template <typename T>
@@ -292,6 +315,7 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
+    a.pref = T(pref);
     return a;
   }
   // forward x pass
@@ -311,6 +335,7 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
+    a.pref = T(pref);
     KScope ks(mode == XF_FLUX3 ? "xfwd_flux" : "xfwd_copy", st);
     if (mode == XF_FLUX3) launch_xfwd<T, XF_FLUX3>(nx, ny, nz, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
@@ -549,6 +574,7 @@ struct Plan2D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
+    a.pref = T(pref);
     return a;
   }
   void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st,
@@ -569,6 +595,7 @@ struct Plan2D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
+    a.pref = T(pref);
     if (mode == XF_FLUX2) launch_xfwd<T, XF_FLUX2>(nx, ny, nb, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nb, a, st);
     check_launch("k_xfwd(2d)");

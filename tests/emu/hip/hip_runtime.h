@@ -61,6 +61,7 @@ void launch(K kernel, dim3 grid, dim3 block, A... args) {
 inline void __syncthreads() { ::emu::bar->arrive_and_wait(); }
 #define __builtin_amdgcn_s_barrier() __syncthreads()
 #define __builtin_amdgcn_fence(...) ((void)0)
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 
 #define hipLaunchKernelGGL(K, G, B, SHM, ST, ...) ::emu::launch(K, G, B, __VA_ARGS__)
 
