@@ -43,6 +43,10 @@ def byte_model(n: int):
         # fused stage boundaries: D in, U in, accumulator, next-stage spectra out
         "xmid1": 5 * a + 5 + 5 + 15 * a, "xmid23": 5 * a + 5 + 10 + 15 * a,
         "xmid_last": 5 * a + 5 + 5 + 5 * a,
+        # FP32 filter in correction form: stage 4 also writes the new state,
+        # unit-factor y pass, (sx sy sz - 1) z pass, x inverse added to it
+        "xmid_last_keep": 5 * a + 5 + 5 + 5 * a + 5, "yfwd_unit": 5 * a + 5 * b,
+        "z_corr": 10 * b, "xinv_add": 5 * a + 5 + 5,
     }
     # per-step model of the unfused pencil pipeline (stage states written and
     # re-read), the official figure for hbm_frac_step
@@ -61,18 +65,27 @@ def kernel_symbol(tag: str, n: int, prec: str) -> str:
             "yinv": f"k_col<{C}, {n}, 4>", "xinv_out": f"k_xinv<{T}, {n}, true, 5, 0>",
             # one kernel for every stage boundary (stage 4 branches at run time)
             "xmid1": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid23": f"k_xmid<{T}, {n}, true, 5, 0>",
-            "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>"}.get(tag, tag)
+            "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid_last_keep": f"k_xmid<{T}, {n}, true, 5, 0>",
+            "yfwd_unit": f"k_col<{C}, {n}, 1>", "z_corr": f"k_col<{C}, {n}, 5>",
+            "xinv_add": f"k_xinv<{T}, {n}, true, 5, 4>"}.get(tag, tag)
 
 
 # launches of each timing tag in one RK4 step of the fused 3D pipeline
 STEP_LAUNCHES = {"xfwd_flux": 1, "yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
                  "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xinv_out": 1}
+# FP32 steps on a previous step's output (the filter in correction form)
+STEP_LAUNCHES_CORR = {"xfwd_flux": 1, "yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
+                      "xmid_last_keep": 1, "yfwd_unit": 1, "z_corr": 1, "xinv_add": 1}
 
 
-def fused_step_units(n: int) -> float:
+def step_launches(prec: str):
+    return STEP_LAUNCHES_CORR if prec == "single" else STEP_LAUNCHES
+
+
+def fused_step_units(n: int, prec: str = "double") -> float:
     """Bytes per step (units of s*N^3) that the fused pipeline moves."""
     per, _ = byte_model(n)
-    return sum(per[t] * k for t, k in STEP_LAUNCHES.items())
+    return sum(per[t] * k for t, k in step_launches(prec).items())
 
 
 def cpu_baseline(n: int, steps: int):
@@ -299,7 +312,7 @@ def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, n, prec, c
     per, _ = byte_model(n)
     alg = {}
     for tag, units in per.items():
-        alg.setdefault(kernel_symbol(tag, n, prec), []).extend([units * es * n ** 3] * STEP_LAUNCHES.get(tag, 1))
+        alg.setdefault(kernel_symbol(tag, n, prec), []).extend([units * es * n ** 3] * step_launches(prec).get(tag, 1))
     acc = {}
     for did, (name, v) in fe.items():
         acc.setdefault(name, [0, 0.0, 0.0])
@@ -487,8 +500,9 @@ def main():
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "hbm_frac_step": round(step_bytes / (ms_per_step / 1e3) / HBM_PEAK, 4),
         "step_model_bytes": int(step_bytes),
-        "step_fused_bytes": int(fused_step_units(n) * es * n ** 3),
-        "hbm_frac_step_fused": round(fused_step_units(n) * es * n ** 3 / (ms_per_step / 1e3) / HBM_PEAK, 4),
+        "step_fused_bytes": int(fused_step_units(n, args.precision) * es * n ** 3),
+        "hbm_frac_step_fused": round(fused_step_units(n, args.precision) * es * n ** 3 / (ms_per_step / 1e3)
+                                     / HBM_PEAK, 4),
         "wall_s": round(wall, 3),
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
         "mass_drift": float(sums[0] / n ** 3 - 1.0),

@@ -69,15 +69,6 @@ int ph_plan3d_create(ph_plan3d_t* plan, int nx, int ny, int nz, double lx, doubl
 int ph_plan3d_destroy(ph_plan3d_t plan);
 int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes);
 
-/* Reference pressure p_ref taken out of the pressure in the diagonal momentum
- * fluxes: F = m u + (p - p_ref) instead of m u + p (phrike/equations.py:
- * 167-196, 119-134).  The spectral derivative of a constant is zero, so the
- * operator is unchanged; in FP32 the fluxes then round on the scale of the
- * pressure variation rather than of its mean.  Default 0, which is the
- * reference's expression.  No reference counterpart: a numerics setting of
- * this backend for its FP32 plans; read at each launch. */
-int ph_plan3d_set_pressure_offset(ph_plan3d_t plan, double p_ref);
-
 /* rhs = -(dx Fx + dy Fy + dz Fz) from U (5 fields).
  * Replaces _compute_rhs_3d (phrike/solver.py:1177-1184) with
  * EulerEquations3D.flux (phrike/equations.py:582-607) and Grid3D.dx1/dy1/dz1
@@ -97,6 +88,18 @@ int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out
  * _apply_physical_filters_3d (phrike/solver.py:1220-1234, 1313-1337). */
 int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev,
               void* ws, void* stream);
+
+/* ph_step3d with flags.  PH_STEP_BANDLIMITED: U has no content beyond the 2/3
+ * cutoff (true of every output of a previous step with the mask on, whose RHS
+ * and filter are both masked).  FP32 plans then apply the post-step filter in
+ * correction form, out = U' + F^-1((sx sy sz - 1) F U') on the kept modes,
+ * which is the same operator on such states; the FP32 transforms then carry
+ * only the small correction instead of the whole state.  FP64 plans ignore
+ * the flag (direct form, the reference's operation order).  No reference
+ * counterpart: a precision choice of this backend's FP32 plans. */
+#define PH_STEP_BANDLIMITED 1
+int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, int flags,
+                 void* ws, void* stream);
 
 /* max over cells of max(|ux|,|uy|,|uz|) + sqrt(gamma p / rho) -> max_dev[0]
  * (FP64), and when dt_dev is not NULL dt_dev[0] = cfl*min(dx,dy,dz)/max (1e-6
@@ -122,8 +125,6 @@ int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, in
                      double gamma, int batch);
 int ph_plan2d_destroy(ph_plan2d_t plan);
 int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes);
-/* As ph_plan3d_set_pressure_offset, for every run of the batch. */
-int ph_plan2d_set_pressure_offset(ph_plan2d_t plan, double p_ref);
 
 /* rhs = -(dx Fx + dy Fy) for the first nbatch runs.  Replaces
  * _compute_rhs_2d (phrike/solver.py:428-443). */

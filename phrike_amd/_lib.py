@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("PHRIKE_HIP_LIB") or os.path.join(os.path.dirname(os.p
 
 PH_F64 = 0
 PH_F32 = 1
+PH_STEP_BANDLIMITED = 1
 
 # symbol -> (restype, argtypes); mirrors include/phrike_hip.h
 _SIGNATURES = {
@@ -26,17 +27,16 @@ _SIGNATURES = {
                                  c_int, c_int, c_int, c_int, c_double, c_double]),
     "ph_plan3d_destroy": (c_int, [c_void_p]),
     "ph_plan3d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
-    "ph_plan3d_set_pressure_offset": (c_int, [c_void_p, c_double]),
     "ph_rhs3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_deriv3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "ph_step3d_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_int, c_void_p, c_void_p]),
     "ph_max_wave_speed3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "ph_conserved_sums3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_plan2d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
                                  c_int, c_int, c_double, c_double, c_int]),
     "ph_plan2d_destroy": (c_int, [c_void_p]),
     "ph_plan2d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
-    "ph_plan2d_set_pressure_offset": (c_int, [c_void_p, c_double]),
     "ph_rhs2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_deriv2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,

@@ -129,13 +129,6 @@ int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes) {
   });
 }
 
-int ph_plan3d_set_pressure_offset(ph_plan3d_t plan, double p_ref) {
-  return guard([&] {
-    need(std::isfinite(p_ref), "pressure offset must be finite");
-    with3(plan, [&](auto& p) { p.pref = p_ref; });
-  });
-}
-
 int ph_rhs3d(ph_plan3d_t plan, const void* U, void* rhs, void* ws, void* stream) {
   return guard([&] {
     need(U && rhs && ws, "null buffer");
@@ -159,16 +152,23 @@ int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out
   });
 }
 
-int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, void* ws,
-              void* stream) {
+int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, int flags,
+                 void* ws, void* stream) {
   return guard([&] {
     need(U && out && ws, "null buffer");
     need(U != out, "U and out must not overlap");
+    need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
-      p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, ws, S(stream));
+      p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, ws, S(stream),
+             (flags & PH_STEP_BANDLIMITED) != 0);
     });
   });
+}
+
+int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, void* ws,
+              void* stream) {
+  return ph_step3d_ex(plan, U, out, dt, dt_dev, 0, ws, stream);
 }
 
 int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double* dt_dev,
@@ -218,13 +218,6 @@ int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes) {
   return guard([&] {
     need(bytes != nullptr, "null output");
     with2(plan, [&](auto& p) { *bytes = p.wsBytes; });
-  });
-}
-
-int ph_plan2d_set_pressure_offset(ph_plan2d_t plan, double p_ref) {
-  return guard([&] {
-    need(std::isfinite(p_ref), "pressure offset must be finite");
-    with2(plan, [&](auto& p) { p.pref = p_ref; });
   });
 }
 

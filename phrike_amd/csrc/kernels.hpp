@@ -294,7 +294,6 @@ struct XFwdArgs {
   const T* sig;  // sigma_x(m) * scale, m < mx
   const T* kk;   // kx(m), zero at the Nyquist mode
   T gamma;
-  T pref;        // constant taken out of the pressure in the momentum fluxes
   const int* active;  // per-slab flag (batched 2D) or null
 };
 
@@ -305,9 +304,7 @@ template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG
 
 // Fluxes (or copies) of the conserved values q (both rows of the thread's x
 // positions) -> forward x transform of two rows at once -> kept modes to
-// a.out.  Shared by k_xfwd and the fused k_xmid.  A constant pref is taken
-// out of the pressure in the diagonal momentum fluxes (its derivative is zero;
-// with pref = 0 the fluxes are the reference's expressions).
+// a.out.  Shared by k_xfwd and the fused k_xmid.
 template <typename T, int NX, bool DA, int MODE>
 __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
                                           const T* sig, const T* kk, int tid, int slab, int y0,
@@ -330,14 +327,14 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
           const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
           const T p = (a.gamma - T(1)) * (u[4] - kin);
           pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = uz;
-          pr[j][r][4] = p - a.pref;
+          pr[j][r][4] = p;
           pr[j][r][5] = u[4] + p;  // E + p
         } else {
           const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri);
           const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
           const T p = (a.gamma - T(1)) * (u[3] - kin);
           pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = T(0);
-          pr[j][r][4] = p - a.pref;
+          pr[j][r][4] = p;
           pr[j][r][5] = u[3] + p;
         }
       }
@@ -431,7 +428,8 @@ enum ColMode {
   CM_FWD_COPY = 1,    // forward, sigma (and i*k), compact out, nc fields
   CM_FWDINV_DIV = 2,  // forward both halves, D = sigma (X + i k Z), inverse, full out
   CM_FWDINV_COPY = 3, // forward, sigma (and i*k) on kept modes, inverse, full out
-  CM_INV = 4          // compact in, zero-filled inverse, full out
+  CM_INV = 4,         // compact in, zero-filled inverse, full out
+  CM_FWDINV_CORR = 5  // forward, (sx sy sz - 1) on kept modes, inverse, full out
 };
 
 template <typename T>
@@ -448,6 +446,11 @@ struct ColArgs {
   const T* sig;  // by compact index
   const T* kk;   // by compact index
   const int* active;  // per blockIdx.y flag (batched 2D runs) or null
+  // CM_FWDINV_CORR: sigma - 1 of this axis (by compact index), of the column
+  // axis (by column) and of the axis of blockIdx.y (by blockIdx.y)
+  const T* sm1;
+  const T* colsm1;
+  const T* rowsm1;
 };
 
 #ifndef PH_COL_ELEMS_F64
@@ -614,7 +617,8 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         }
       });
     } else {
-      // CM_FWDINV_DIV / CM_FWDINV_COPY: forward, combine on kept modes, inverse
+      // CM_FWDINV_DIV / CM_FWDINV_COPY / CM_FWDINV_CORR: forward, combine on
+      // kept modes, inverse
       fwd(FWD{});
       if constexpr (MODE == CM_FWDINV_DIV) {
         keep_first();
@@ -626,6 +630,11 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           const int j = kp.compact(n, N);
           if constexpr (MODE == CM_FWDINV_DIV) {
             v = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+          } else if constexpr (MODE == CM_FWDINV_CORR) {
+            // sx sy sz - 1 from the three (sigma - 1): no cancellation
+            const T sa = a.colsm1[c0 + l], sb = a.rowsm1[blockIdx.y], sc = a.sm1[j];
+            const T f = ((sa + sb) + sc) + ((sa * sb + sa * sc) + sb * sc) + (sa * sb) * sc;
+            v = cscale(X[q][r], f);
           } else {
             v = cscale(X[q][r], a.sig[j]);
             if (a.ikflag) v = mul_ik(v, a.kk[j]);
@@ -649,7 +658,8 @@ enum XInvMode {
   XI_OUT = 0,      // out = value
   XI_RHS = 1,      // out = -value
   XI_STAGE3 = 2,   // 3D RK4 stage combine
-  XI_STAGE2 = 3    // 2D RK4 stage combine with positivity clamp
+  XI_STAGE2 = 3,   // 2D RK4 stage combine with positivity clamp
+  XI_ADD = 4       // out += value
 };
 
 template <typename T>
@@ -664,7 +674,9 @@ struct XInvArgs {
   long long fs, ss;        // physical field / slab strides (elements); rows of nx
   int stage;               // 1..4
   double dt;               // used when dt_dev is null
-  const double* dt_dev;    // per-slab dt (batched 2D) or null
+  const double* dt_dev;    // device dt: dt_dev[slab] (dt_per_slab) or dt_dev[0]; or null
+  int dt_per_slab;         // 1: one dt per slab (batched 2D runs)
+  int keep_state;          // k_xmid stage 4: also write the new state to out
   const int* active;       // per-slab flag (batched 2D) or null
   T gamma, rho_min, p_min;
   const cx<T>* tw;
@@ -732,14 +744,14 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   if (a.active != nullptr && a.active[slab] == 0) return;
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
-  const int nfl = (MODE == XI_OUT || MODE == XI_RHS) ? a.nf : NC;
+  const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
   XSpec<T, NX, DA, NC> sp;
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
   lds_barrier();
   fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
 #pragma unroll
   for (int j = 0; j < XJ; ++j) {
     const int x = tid + j * NT;
@@ -762,6 +774,10 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           if (c < nfl) a.out[c * a.fs + o] = -v[r][c];
+      } else if constexpr (MODE == XI_ADD) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          if (c < nfl) a.out[c * a.fs + o] = a.out[c * a.fs + o] + v[r][c];
       } else {
         T un[NC], acc[NC], u[NC];
         rk_inputs<T, NC>(a, o, un, acc);
@@ -777,7 +793,9 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
 // Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
 // the fluxes of the stage s+1 state and their forward x transform, or (stage
 // 4) the new state's forward x transform for the post-step filter, which
-// always follows and writes the state itself.  The intermediate stage states
+// always follows and writes the state itself (with keep_state the new state
+// is written here too: the filter then adds its correction to it).  The
+// intermediate stage states
 // never go to HBM.  The spectra are read from and written to the same rows of
 // the same buffer; the block loads its rows completely before it writes them.
 template <typename T>
@@ -806,7 +824,9 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   XSpec<T, NX, DA, NC> sp;
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, NC, tid);
   PH_XTW_FILL(a.tw, NX, NT, tid);
-  x_tables<T, NX, DA>(m.fwd.sig, m.fwd.kk, s_sig, s_k, tid);
+  // the forward that follows: stages 1-3 fluxes, stage 4 the new state
+  const XFwdArgs<T>& fa = a.stage < 4 ? m.fwd : m.fwd_last;
+  x_tables<T, NX, DA>(fa.sig, fa.kk, s_sig, s_k, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
   // the RK combine's inputs, loaded before the inverse transform so their
   // latency overlaps it (PH_XMID_PREFETCH)
@@ -833,7 +853,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
   lds_barrier();
   fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[slab] : a.dt;
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
   T q[XJ][2][5];
 #pragma unroll
   for (int j = 0; j < XJ; ++j) {
@@ -866,6 +886,10 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
       rk_point<T, NC, NC == 4>(a, v[r], o, dtd, u, un, acc);
 #pragma unroll
       for (int c = 0; c < NC; ++c) q[j][r][c] = u[c];
+      if (a.stage == 4 && a.keep_state) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
+      }
     }
   }
   lds_barrier();  // buf is reused by the forward transform

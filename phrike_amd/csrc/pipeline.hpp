@@ -31,7 +31,7 @@ inline bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 struct AxisTables {
   int n = 0;
   int kp = 0, kn = 0, nr = 0;  // kept positions n <= kp and n >= N-kn
-  std::vector<double> k, sig;  // by compact index (x: m = 0..kp)
+  std::vector<double> k, sig, sm1;  // by compact index (x: m = 0..kp); sm1 = sig - 1
   std::vector<double> twr, twi;
 
   AxisTables() = default;
@@ -54,9 +54,14 @@ struct AxisTables {
       double kv = 2.0 * M_PI * (double(m) * inv);
       if (2 * nat == N) kv = 0.0;  // Nyquist: i*k gives no real part
       k.push_back(kv);
-      double s = 1.0;
-      if (filt) s = std::exp(-alpha * std::pow(std::fabs(double(m)) / mmax, double(p)));
+      double s = 1.0, s1 = 0.0;
+      if (filt) {
+        const double g = -alpha * std::pow(std::fabs(double(m)) / mmax, double(p));
+        s = std::exp(g);
+        s1 = std::expm1(g);
+      }
       sig.push_back(s);
+      sm1.push_back(s1);
     }
     twr.resize(N);
     twi.resize(N);
@@ -74,20 +79,28 @@ struct DevAxis {
   cx<T>* tw = nullptr;
   T* sig = nullptr;
   T* k = nullptr;
+  T* one = nullptr;  // the scale alone (no filter factor), by compact index
+  T* sm1 = nullptr;  // sigma - 1, by compact index
   void upload(const AxisTables& a, double scale) {
     std::vector<cx<T>> tw_h(a.n);
     for (int m = 0; m < a.n; ++m) {
       tw_h[m].x = T(a.twr[m]);
       tw_h[m].y = T(a.twi[m]);
     }
-    std::vector<T> s_h(a.nr), k_h(a.nr);
+    std::vector<T> s_h(a.nr), k_h(a.nr), o_h(a.nr), m_h(a.nr);
     for (int j = 0; j < a.nr; ++j) {
       s_h[j] = T(a.sig[j] * scale);
       k_h[j] = T(a.k[j]);
+      o_h[j] = T(scale);
+      m_h[j] = T(a.sm1[j]);
     }
     hip_check(hipMalloc(&tw, sizeof(cx<T>) * a.n), "hipMalloc(twiddles)");
     hip_check(hipMalloc(&sig, sizeof(T) * a.nr), "hipMalloc(sigma)");
     hip_check(hipMalloc(&k, sizeof(T) * a.nr), "hipMalloc(k)");
+    hip_check(hipMalloc(&one, sizeof(T) * a.nr), "hipMalloc(scale)");
+    hip_check(hipMalloc(&sm1, sizeof(T) * a.nr), "hipMalloc(sigma-1)");
+    hip_check(hipMemcpy(one, o_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy scale");
+    hip_check(hipMemcpy(sm1, m_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy sigma-1");
     hip_check(hipMemcpy(tw, tw_h.data(), sizeof(cx<T>) * a.n, hipMemcpyHostToDevice), "copy twiddles");
     hip_check(hipMemcpy(sig, s_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy sigma");
     hip_check(hipMemcpy(k, k_h.data(), sizeof(T) * a.nr, hipMemcpyHostToDevice), "copy k");
@@ -96,9 +109,13 @@ struct DevAxis {
     if (tw) (void)hipFree(tw);
     if (sig) (void)hipFree(sig);
     if (k) (void)hipFree(k);
+    if (one) (void)hipFree(one);
+    if (sm1) (void)hipFree(sm1);
     tw = nullptr;
     sig = nullptr;
     k = nullptr;
+    one = nullptr;
+    sm1 = nullptr;
   }
 };
 
@@ -238,9 +255,6 @@ struct KScope {
 struct PlanBase {
   virtual ~PlanBase() = default;
   int dtype = 0;  // 0 = float64, 1 = float32
-  // constant taken out of the pressure in the momentum fluxes (0: the
-  // reference's expressions); set between calls, read at launch
-  double pref = 0.0;
 };
 
 template <typename T>
@@ -254,6 +268,7 @@ struct Plan3D : PlanBase {
   DevAxis<CE> dy, dz;
   int mx, mxp;
   long long ncell;
+  CE* xcol_sm1 = nullptr;  // sigma_x - 1 per column of the column passes
   // workspace layout (bytes)
   size_t offA, offB, offUs, offPart, wsBytes;
   static constexpr int NPART = 1024;
@@ -275,6 +290,20 @@ struct Plan3D : PlanBase {
     dx.upload(ax, scale);
     dy.upload(ay, 1.0);
     dz.upload(az, 1.0);
+    {
+      const int ncol = (mx + PW - 1) / PW;
+      std::vector<CE> h(ncol);
+      for (int c = 0; c < ncol; ++c) {
+        if constexpr (PW == 2) {
+          const double v0 = ax.sm1[2 * c], v1 = 2 * c + 1 < mx ? ax.sm1[2 * c + 1] : 0.0;
+          h[c] = CE(float(v0), float(v1));
+        } else {
+          h[c] = CE(ax.sm1[c]);
+        }
+      }
+      hip_check(hipMalloc(&xcol_sm1, sizeof(CE) * ncol), "hipMalloc(column sigma-1)");
+      hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncol, hipMemcpyHostToDevice), "copy column sigma-1");
+    }
     const size_t cs = sizeof(cx<T>);
     const size_t szA = 15ull * nz * ny * mxp * cs;
     const size_t szB = 10ull * nz * ay.nr * mxp * cs;
@@ -286,6 +315,7 @@ struct Plan3D : PlanBase {
     wsBytes = al(offPart + sizeof(double) * NPART * 5);
   }
   ~Plan3D() override {
+    if (xcol_sm1) (void)hipFree(xcol_sm1);
     dx.release();
     dy.release();
     dz.release();
@@ -315,7 +345,6 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
-    a.pref = T(pref);
     return a;
   }
   // forward x pass
@@ -335,14 +364,13 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
-    a.pref = T(pref);
     KScope ks(mode == XF_FLUX3 ? "xfwd_flux" : "xfwd_copy", st);
     if (mode == XF_FLUX3) launch_xfwd<T, XF_FLUX3>(nx, ny, nz, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
     check_launch("k_xfwd");
   }
   // y forward: A -> B
-  void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
+  void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st, bool unit = false) const {
     ColArgs<CE> a{};
     a.in = reinterpret_cast<cx<CE>*>(A(ws));
     a.in_fs = (a_fs()) / PW;
@@ -357,9 +385,9 @@ struct Plan3D : PlanBase {
     a.keep = ay.keep();
     a.ikflag = ikflag;
     a.tw = dy.tw;
-    a.sig = dy.sig;
+    a.sig = unit ? dy.one : dy.sig;
     a.kk = dy.k;
-    KScope ks(div ? "yfwd_div" : "yfwd_copy", st);
+    KScope ks(div ? "yfwd_div" : (unit ? "yfwd_unit" : "yfwd_copy"), st);
     if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
     else launch_col<CE, CM_FWD_COPY>(ny, nz, a, st);
     check_launch("k_col(y fwd)");
@@ -386,6 +414,31 @@ struct Plan3D : PlanBase {
     if (div) launch_col<CE, CM_FWDINV_DIV>(nz, ay.nr, a, st);
     else launch_col<CE, CM_FWDINV_COPY>(nz, ay.nr, a, st);
     check_launch("k_col(z)");
+  }
+  // filter correction along z: kept modes times sx sy sz - 1 (after unit x
+  // and y passes), in place in B
+  void zcorr(int nc, void* ws, hipStream_t st) const {
+    ColArgs<CE> a{};
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = (mxp) / PW;
+    a.in_ls = ((long long)ay.nr * mxp) / PW;
+    a.out = reinterpret_cast<cx<CE>*>(B(ws));
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = (mx + PW - 1) / PW;
+    a.nc = nc;
+    a.keep = az.keep();
+    a.tw = dz.tw;
+    a.sig = dz.sig;
+    a.kk = dz.k;
+    a.sm1 = dz.sm1;
+    a.colsm1 = xcol_sm1;
+    a.rowsm1 = dy.sm1;
+    KScope ks("z_corr", st);
+    launch_col<CE, CM_FWDINV_CORR>(nz, ay.nr, a, st);
+    check_launch("k_col(z corr)");
   }
   // y inverse: B (compact) -> A (full)
   void yinv(int nc, void* ws, hipStream_t st) const {
@@ -457,7 +510,16 @@ struct Plan3D : PlanBase {
   // RK4 (phrike/solver.py:1220-1230) then the post-step filter (1233-1234,
   // 1336).  `out` doubles as the stage accumulator; each stage boundary is
   // one fused k_xmid launch (x inverse + combine + next fluxes + x forward).
-  void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st) const {
+  // `bandlimited`: U has no content beyond the 2/3 cutoff (it is the output of
+  // a previous step, whose RHS and filter are both masked).  FP32 plans then
+  // apply the post-step filter in correction form, out = U' + F^-1((sx sy sz
+  // - 1) F U') on the kept modes: the same operator on such states, but the
+  // FP32 transforms carry only the small correction instead of the whole
+  // state, whose round-trip rounding would otherwise accumulate step after
+  // step.  FP64 plans always use the direct form (the reference's order).
+  void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st,
+            bool bandlimited = false) const {
+    const bool corr = bandlimited && sizeof(T) == 4;
     xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st);
     for (int s = 1; s <= 4; ++s) {
       yfwd(true, 5, 0, ws, st);
@@ -471,23 +533,34 @@ struct Plan3D : PlanBase {
       m.inv.stage = s;
       m.inv.dt = dt;
       m.inv.dt_dev = dt_dev;
+      m.inv.keep_state = corr ? 1 : 0;
       m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws);
       m.fwd_last = xfwd_args(nullptr, 5, 0u, ws);
-      KScope ks(s == 4 ? "xmid_last" : (s == 1 ? "xmid1" : "xmid23"), st);
+      if (corr) m.fwd_last.sig = dx.one;
+      KScope ks(s == 4 ? (corr ? "xmid_last_keep" : "xmid_last") : (s == 1 ? "xmid1" : "xmid23"), st);
       launch_xmid<T, 5, XF_FLUX3>(nx, ny, nz, m, st);
       check_launch("k_xmid");
     }
     // remaining passes of the post-step filter on the new state
-    yfwd(false, 5, 0, ws, st);
-    zpass(false, 5, 0, ws, st);
-    yinv(5, ws, st);
     XInvArgs<T> a = xinv_args(ws);
     a.nf = 5;
     a.out = out;
-    KScope ks("xinv_out", st);
-    launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+    if (corr) {
+      yfwd(false, 5, 0, ws, st, true);
+      zcorr(5, ws, st);
+      yinv(5, ws, st);
+      KScope ks("xinv_add", st);
+      launch_xinv<T, 5, XI_ADD>(nx, ny, nz, a, st);
+    } else {
+      yfwd(false, 5, 0, ws, st);
+      zpass(false, 5, 0, ws, st);
+      yinv(5, ws, st);
+      KScope ks("xinv_out", st);
+      launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+    }
     check_launch("k_xinv");
   }
+
   void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
@@ -574,7 +647,6 @@ struct Plan2D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
-    a.pref = T(pref);
     return a;
   }
   void xfwd(int mode, const T* in, int nb, unsigned ikmask, void* ws, hipStream_t st,
@@ -595,7 +667,6 @@ struct Plan2D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
-    a.pref = T(pref);
     if (mode == XF_FLUX2) launch_xfwd<T, XF_FLUX2>(nx, ny, nb, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nb, a, st);
     check_launch("k_xfwd(2d)");
@@ -672,6 +743,7 @@ struct Plan2D : PlanBase {
       m.inv.stage = s;
       m.inv.dt = dt;
       m.inv.dt_dev = dt_dev;
+      m.inv.dt_per_slab = 1;
       m.inv.active = active;
       if (s == 4 && !filter) {
         launch_xinv<T, 4, XI_STAGE2>(nx, ny, nb, m.inv, st);

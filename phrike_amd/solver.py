@@ -8,6 +8,7 @@ GPU stream.  The CFL reduction and the conservation sums are HIP kernels too.
 """
 from __future__ import annotations
 
+import weakref
 from typing import Any, Callable, Dict, List, Optional
 
 import numpy as np
@@ -50,27 +51,12 @@ class SpectralSolver3D:
         self.gravity_config = gravity_config
         self._plan = grid._plan(equations.gamma)
         self._scalars = torch.zeros(8, dtype=torch.float64, device=grid.device)
-        self._pref_set = grid.dtype != torch.float32
+        self._last_out = None  # weak reference to the last step's output
+        self._last_version = -1
 
     def _ctx(self):
         return self._plan, _lib.ptr(self.grid.workspace(self.equations.gamma)), \
             _lib.stream_handle(torch, self.grid.device)
-
-    def _pressure_offset(self, U) -> None:
-        """FP32 plans: once, take the mean pressure of the state (FP64, from
-        a stride-4 subsample) out of the pressure in the momentum fluxes
-        (ph_plan3d_set_pressure_offset).  Its derivative is zero, so the
-        operator is the same; the FP32 fluxes then round on the scale of the
-        pressure variation.  FP64 plans keep the reference's expressions."""
-        if self._pref_set:
-            return
-        gm1 = self.equations.gamma - 1.0
-        s = U[:, ::4, ::4, ::4].double()
-        p = gm1 * (s[4] - 0.5 * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) / s[0])
-        pref = float(p.mean().item())
-        if np.isfinite(pref):
-            _lib.call("ph_plan3d_set_pressure_offset", self._plan, pref)
-        self._pref_set = True
 
     def _state(self, U):
         g = self.grid
@@ -105,14 +91,21 @@ class SpectralSolver3D:
     _fixed_step = step
 
     def step_into(self, U, out, dt: float, dt_dev=None) -> None:
-        """out = filter(RK4(U, dt)) with caller-owned output (no allocation)."""
-        self._pressure_offset(U)
+        """out = filter(RK4(U, dt)) with caller-owned output (no allocation).
+
+        When U is this solver's own previous output, unmodified since (same
+        tensor object, same version counter), it has no content beyond the 2/3
+        cutoff and the step says so (PH_STEP_BANDLIMITED): FP32 plans then
+        filter in correction form.  Any other input takes the direct form."""
         plan, ws, st = self._ctx()
-        _lib.call("ph_step3d", plan, _lib.ptr(U), _lib.ptr(out), float(dt), _lib.ptr(dt_dev), ws, st)
+        last = self._last_out() if self._last_out is not None else None
+        flags = _lib.PH_STEP_BANDLIMITED if (last is U and U._version == self._last_version) else 0
+        _lib.call("ph_step3d_ex", plan, _lib.ptr(U), _lib.ptr(out), float(dt), _lib.ptr(dt_dev), flags, ws, st)
+        self._last_out = weakref.ref(out)
+        self._last_version = out._version
 
     def rhs(self, U):
         U = self._state(U)
-        self._pressure_offset(U)
         out = torch.empty_like(U)
         plan, ws, st = self._ctx()
         _lib.call("ph_rhs3d", plan, _lib.ptr(U), _lib.ptr(out), ws, st)

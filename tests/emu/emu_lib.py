@@ -56,10 +56,10 @@ class Emu3D:
         self._chk(self.lib.ph_rhs3d(self.h, P(U), P(out), P(self.ws), None))
         return out
 
-    def step(self, U, dt):
+    def step(self, U, dt, flags=0, dt_dev=None):
         U = np.ascontiguousarray(U, dtype=self.dtype)
         out = np.zeros_like(U)
-        self._chk(self.lib.ph_step3d(self.h, P(U), P(out), dt, None, P(self.ws), None))
+        self._chk(self.lib.ph_step3d_ex(self.h, P(U), P(out), dt, P(dt_dev), flags, P(self.ws), None))
         return out
 
 

@@ -38,6 +38,30 @@ def test_cube_step_fp32(emu):
     assert group_errors(got, d["cube16_step"], [(0,), (1, 2, 3), (4,)]).max() < 1e-5
 
 
+def test_fp32_filter_correction_form(emu):
+    """FP32 step on a band-limited state (a previous step's output) whose kept
+    modes carry a flat random spectrum, so the filter acts on every mode: the
+    correction-form filter equals the direct form up to FP32 rounding; FP64
+    plans ignore the flag; a device dt (dt_dev[0]) equals the host dt."""
+    from emu_lib import Emu3D
+    d = golden("tgv3d_16.npz")
+    dt = float(d["dt"])
+    rng = np.random.default_rng(3)
+    U0 = d["U0"] + 0.05 * rng.standard_normal(d["U0"].shape) * np.array([1, 1, 1, 1, 10.0])[:, None, None, None]
+    e64 = Emu3D(emu, 16, 16, 16)
+    V1 = e64.step(U0, dt)  # band-limited
+    e32 = Emu3D(emu, 16, 16, 16, dtype=np.float32)
+    direct = e32.step(V1, dt).astype(np.float64)
+    corr = e32.step(V1, dt, flags=1).astype(np.float64)
+    ref = e64.step(V1, dt)
+    print("corr vs direct", group_errors(corr, direct, [(0,), (1, 2, 3), (4,)]),
+          "corr vs fp64", group_errors(corr, ref, [(0,), (1, 2, 3), (4,)]))
+    assert group_errors(corr, ref, [(0,), (1, 2, 3), (4,)]).max() < 1e-6
+    assert group_errors(direct, ref, [(0,), (1, 2, 3), (4,)]).max() < 1e-6
+    np.testing.assert_array_equal(e64.step(V1, dt, flags=1), ref)
+    np.testing.assert_array_equal(e64.step(V1, 0.0, dt_dev=np.array([dt])), ref)
+
+
 def test_khi_rect_step_and_batch(emu):
     from emu_lib import Emu2D
     d = golden("khi2d.npz")
