@@ -38,14 +38,15 @@ def byte_model(n: int):
     per = {
         "xfwd_flux": 5 + 15 * a, "yfwd_div": 15 * a + 10 * b, "z_div": 15 * b,
         "yinv": 5 * b + 5 * a, "xinv_stage1": 5 * a + 15, "xinv_stage23": 5 * a + 20,
-        "xinv_stage4": 5 * a + 15, "xfwd_copy": 5 + 5 * a, "yfwd_copy": 5 * a + 5 * b,
+        "xinv_stage4": 5 * a + 10, "xfwd_copy": 5 + 5 * a, "yfwd_copy": 5 * a + 5 * b,
         "z_copy": 10 * b, "xinv_out": 5 * a + 5,
         # fused stage boundaries: D in, U in, accumulator, next-stage spectra out
         "xmid1": 5 * a + 5 + 5 + 15 * a, "xmid23": 5 * a + 5 + 10 + 15 * a,
-        "xmid_last": 5 * a + 5 + 5 + 5 * a,
+        # (stage 4 reads the accumulator, which carries U, and not U itself)
+        "xmid_last": 5 * a + 5 + 5 * a,
         # FP32 filter in correction form: stage 4 also writes the new state,
         # unit-factor y pass, (sx sy sz - 1) z pass, x inverse added to it
-        "xmid_last_keep": 5 * a + 5 + 5 + 5 * a + 5, "yfwd_unit": 5 * a + 5 * b,
+        "xmid_last_keep": 5 * a + 5 + 5 * a + 5, "yfwd_unit": 5 * a + 5 * b,
         "z_corr": 10 * b, "xinv_add": 5 * a + 5 + 5,
     }
     # per-step model of the unfused pencil pipeline (stage states written and

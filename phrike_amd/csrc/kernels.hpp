@@ -468,6 +468,9 @@ struct ColArgs {
 // Two-input modes hold the first spectrum in registers (YREG) or in a second
 // LDS array (default: measured equal in FP32 at 8-column tiles, and the
 // registers spill in FP64 or at 16 FP32 columns).
+#ifndef PH_COL_TM_MAX
+#define PH_COL_TM_MAX 8
+#endif
 #ifndef PH_COL_PER_THREAD_F32
 #define PH_COL_PER_THREAD_F32 8
 #endif
@@ -484,7 +487,9 @@ struct ColTile {
   static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
   static constexpr int ELEMS =
       (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : (PAIR ? PH_COL_ELEMS_F32 : PH_COL_ELEMS_F32_SINGLE);
-  static constexpr int TM = clampi(ELEMS / N, 1, 64);
+  // columns per tile: PH_COL_TM_MAX 16-byte elements at most (128-byte rows),
+  // so short lines make more, smaller blocks rather than wider rows
+  static constexpr int TM = clampi(ELEMS / N < PH_COL_TM_MAX ? ELEMS / N : PH_COL_TM_MAX, 1, 64);
   // complex values of each register array per thread
   static constexpr int PERT = (sizeof(T) == 8) ? 8 : PH_COL_PER_THREAD_F32;
   static constexpr int NT = clampi((N * TM) / PERT, 64, 1024);
@@ -698,37 +703,39 @@ __device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
 }
 
 // RK4 stage combine at one point (phrike/solver.py:1220-1230 3D, 502-513 2D)
-// with k_s = -v: stages 1-3 update the accumulator a.out and return the next
-// stage state in u; stage 4 returns U + dt/6 (acc + k4) in u.  The 2D form
-// clamps u (phrike/solver.py:467-488).
+// with k_s = -v.  The accumulator a.out carries the step's sum so far,
+// acc = U + dt/6 k1 (stage 1), + dt/3 k2, + dt/3 k3 (stages 2, 3); stages 1-3
+// return the next stage state U + c k_s in u, stage 4 returns acc + dt/6 k4:
+// the reference's U + dt/6 (k1 + 2 k2 + 2 k3 + k4), summed in another order
+// so that stage 4 needs no U.  The 2D form clamps u (phrike/solver.py:467-488).
 template <typename T, int NC, bool CLAMP>
 __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC], long long o, double dtd,
                                          T (&u)[NC], const T (&un)[NC], const T (&acc)[NC]) {
-  const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0);
+  const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0), third = T(dtd / 3.0);
   if (a.stage < 4) {
     const T coef = (a.stage == 3) ? full : half;
+    const T w = (a.stage == 1) ? sixth : third;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const T k = -v[c];
       const long long idx = c * a.fs + o;
-      if (a.stage == 1) a.out[idx] = k;
-      else a.out[idx] = acc[c] + T(2) * k;
+      a.out[idx] = (a.stage == 1 ? un[c] : acc[c]) + w * k;
       u[c] = un[c] + coef * k;
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) u[c] = un[c] + sixth * (acc[c] + (-v[c]));
+    for (int c = 0; c < NC; ++c) u[c] = acc[c] + sixth * (-v[c]);
   }
   if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
 }
 
-// the RK combine's inputs at one point: U at the start of the step and (from
-// stage 2 on) the accumulator
+// the RK combine's inputs at one point: U (stages 1-3) and the accumulator
+// (stages 2-4)
 template <typename T, int NC>
 __device__ __forceinline__ void rk_inputs(const XInvArgs<T>& a, long long o, T (&un)[NC], T (&acc)[NC]) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    un[c] = a.un[c * a.fs + o];
+    un[c] = a.stage < 4 ? a.un[c * a.fs + o] : T(0);
     acc[c] = a.stage > 1 ? a.out[c * a.fs + o] : T(0);
   }
 }
@@ -844,7 +851,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
           pre_acc[j][r][c] = T(0);
           if (G::XALL || x < NX) {
             const long long o = base + r * NX + x;
-            pre_un[j][r][c] = a.un[c * a.fs + o];
+            if (a.stage < 4) pre_un[j][r][c] = a.un[c * a.fs + o];
             if (PH_XMID_PREFETCH == 1 && a.stage > 1) pre_acc[j][r][c] = a.out[c * a.fs + o];
           }
         }

@@ -12,6 +12,10 @@
 
 #include "kernels.hpp"
 
+#ifndef PH_B_MYZ
+#define PH_B_MYZ 0
+#endif
+
 namespace ph {
 
 struct Error : std::runtime_error {
@@ -328,6 +332,11 @@ struct Plan3D : PlanBase {
 
   long long a_fs() const { return (long long)nz * ny * mxp; }
   long long b_fs() const { return (long long)nz * ay.nr * mxp; }
+  // B strides (complex values) between z planes and between kept y modes:
+  // [field][z][my][mx] by default, [field][my][z][mx] with PH_B_MYZ (z lines
+  // then contiguous rows of mxp)
+  long long b_zs() const { return PH_B_MYZ ? (long long)mxp : (long long)ay.nr * mxp; }
+  long long b_ys() const { return PH_B_MYZ ? (long long)nz * mxp : (long long)mxp; }
 
   XFwdArgs<T> xfwd_args(const T* in, int nf, unsigned ikmask, void* ws) const {
     XFwdArgs<T> a{};
@@ -378,8 +387,8 @@ struct Plan3D : PlanBase {
     a.in_ls = (mxp) / PW;
     a.out = reinterpret_cast<cx<CE>*>(B(ws));
     a.out_fs = (b_fs()) / PW;
-    a.out_bs = ((long long)ay.nr * mxp) / PW;
-    a.out_ls = (mxp) / PW;
+    a.out_bs = b_zs() / PW;
+    a.out_ls = b_ys() / PW;
     a.ncol = (mx + PW - 1) / PW;
     a.nc = nc;
     a.keep = ay.keep();
@@ -397,8 +406,8 @@ struct Plan3D : PlanBase {
     ColArgs<CE> a{};
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
-    a.in_bs = (mxp) / PW;
-    a.in_ls = ((long long)ay.nr * mxp) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
     a.out = reinterpret_cast<cx<CE>*>(B(ws));
     a.out_fs = a.in_fs;
     a.out_bs = a.in_bs;
@@ -421,8 +430,8 @@ struct Plan3D : PlanBase {
     ColArgs<CE> a{};
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
-    a.in_bs = (mxp) / PW;
-    a.in_ls = ((long long)ay.nr * mxp) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
     a.out = reinterpret_cast<cx<CE>*>(B(ws));
     a.out_fs = a.in_fs;
     a.out_bs = a.in_bs;
@@ -445,8 +454,8 @@ struct Plan3D : PlanBase {
     ColArgs<CE> a{};
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
-    a.in_bs = ((long long)ay.nr * mxp) / PW;
-    a.in_ls = (mxp) / PW;
+    a.in_bs = b_zs() / PW;
+    a.in_ls = b_ys() / PW;
     a.out = reinterpret_cast<cx<CE>*>(A(ws));
     a.out_fs = (a_fs()) / PW;
     a.out_bs = ((long long)ny * mxp) / PW;
