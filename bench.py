@@ -347,7 +347,9 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP32-vs-FP64 turbulence study")
-    ap.add_argument("--precision-size", type=int, default=256)
+    ap.add_argument("--precision-size", type=int, default=512)
+    ap.add_argument("--precision-size2", type=int, default=256,
+                    help="second size of the FP32-vs-FP64 study (0: none)")
     ap.add_argument("--precision-steps", type=int, default=200)
     ap.add_argument("--hybrid-steps", type=int, default=1000)
     ap.add_argument("--ensemble-window", type=float, default=3.0)
@@ -478,6 +480,10 @@ def main():
         torch.cuda.empty_cache()
         extras["fp32_vs_fp64"] = precision_study(args.precision_size, args.precision_steps, dev)
         torch.cuda.empty_cache()
+        if args.precision_size2 > 0:
+            extras[f"fp32_vs_fp64_{args.precision_size2}"] = precision_study(args.precision_size2,
+                                                                             args.precision_steps, dev)
+            torch.cuda.empty_cache()
         if args.hybrid_steps > 0:
             print(f"[bench] hybrid KHI: {args.hybrid_steps} steps", file=sys.stderr, flush=True)
             extras["hybrid_khi"] = hybrid_study(256, (128, 256), args.hybrid_steps, dev)

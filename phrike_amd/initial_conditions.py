@@ -12,6 +12,8 @@ with x fastest; conservative components are [rho, rho*u, rho*v, (rho*w,) E].
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 
@@ -60,18 +62,29 @@ def decaying_turbulence_3d(n: int, seed: int = 1234, density_seed: int = 4321,
     (u_hat -= k (k.u_hat)/k^2, as phrike/grid.py:1648-1684), mean removed."""
     rng = np.random.default_rng(seed)
     shape = (n, n, n)
+    try:  # multi-threaded transforms for large grids (host-side set-up only)
+        import scipy.fft as _fft
+        kw = {"workers": os.cpu_count() or 1}
+    except ImportError:  # pragma: no cover
+        _fft, kw = np.fft, {}
     k1 = np.fft.fftfreq(n, d=length / n) * 2 * np.pi
-    KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
-    k2 = KX ** 2 + KY ** 2 + KZ ** 2
+    kh = np.fft.rfftfreq(n, d=length / n) * 2 * np.pi  # last axis: half spectrum
+    KZ, KY, KX = k1[:, None, None], k1[None, :, None], kh[None, None, :]
+    k2 = (KX ** 2 + KY ** 2) + KZ ** 2
     kmag = np.sqrt(k2)
-    spec = kmag ** 4 * np.exp(-2.0 * (kmag / k_peak) ** 2)
     with np.errstate(divide="ignore", invalid="ignore"):
-        amp = np.where(kmag > 0, np.sqrt(spec / (4 * np.pi * np.maximum(k2, 1e-300))), 0.0)
-    uh = [np.fft.fftn(rng.standard_normal(shape)) * amp for _ in range(3)]
+        amp = np.where(kmag > 0, np.sqrt(kmag ** 4 * np.exp(-2.0 * (kmag / k_peak) ** 2)
+                                         / (4 * np.pi * np.maximum(k2, 1e-300))), 0.0)
+    del kmag
+    uh = [_fft.rfftn(rng.standard_normal(shape), **kw) * amp for _ in range(3)]
+    del amp
     k2s = np.where(k2 == 0, 1.0, k2)
     div = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / k2s
+    del k2, k2s
     uh = [uh[0] - KX * div, uh[1] - KY * div, uh[2] - KZ * div]
-    u = [np.fft.ifftn(c).real for c in uh]
+    del div
+    u = [_fft.irfftn(c, s=shape, **kw) for c in uh]
+    del uh
     u = [c - c.mean() for c in u]
     rms = np.sqrt(np.mean(u[0] ** 2 + u[1] ** 2 + u[2] ** 2))
     p0 = 1.0 / gamma
