@@ -167,6 +167,77 @@ def precision_study(n: int, steps: int, dev):
             "energy_drift_f32": float(f"{(f1[4] - f0[4]) / f0[4]:.3e}")}
 
 
+def torch_backend_reference(n: int, steps: int, dev, dt: float, hip_state=None):
+    """The reference's own GPU path (its `torch` backend: phrike/grid.py:1424-1518,
+    1705-1726; phrike/equations.py:582-607; phrike/solver.py:1177-1234), restated
+    with torch.fft (the vendor FFT library) on this GPU, FP64, TGV n^3: full
+    complex fftn of all 5 fields per derivative, full 3D mask and sigma
+    tensors, ifftn(...).real, RK4 with state-sized temporaries, post-step
+    filter.  A second GPU reference point, timed over `steps` steps after one
+    warm-up step; with `hip_state` (this backend's result after the same steps
+    from the same start) the relative L2 difference is reported too."""
+    import torch
+    from phrike_amd.initial_conditions import taylor_green_3d_torch
+
+    gam = 1.4
+    L = 2 * np.pi
+    d = L / n
+    m = np.fft.fftfreq(n, 1.0 / n)
+    k = 2 * np.pi * (m * (1.0 / (n * d)))
+    mask1 = (np.abs(m) <= n // 3).astype(np.float64)
+    sig1 = np.exp(-36.0 * (np.abs(m) / max(n // 2, 1)) ** 16)
+    t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev)
+    mask = t(mask1[:, None, None] * mask1[None, :, None] * mask1[None, None, :])
+    sigma = t((sig1[:, None, None] * sig1[None, :, None]) * sig1[None, None, :])
+    kt = t(k).to(torch.complex128)
+    ikx, iky, ikz = 1j * kt[None, None, :], 1j * kt[None, :, None], 1j * kt[:, None, None]
+    dims = (-3, -2, -1)
+
+    def masked(F):
+        F *= mask
+        F *= sigma
+        return F
+
+    def deriv(f, ik):
+        return torch.fft.ifftn(masked(torch.fft.fftn(f, dim=dims)) * ik, dim=dims).real
+
+    def rhs(U):
+        rho = U[0]
+        ux, uy, uz = U[1] / rho, U[2] / rho, U[3] / rho
+        p = (gam - 1.0) * (U[4] - 0.5 * rho * (ux * ux + uy * uy + uz * uz))
+        mx, my, mz = rho * ux, rho * uy, rho * uz
+        h = U[4] + p
+        Fx = torch.stack([mx, mx * ux + p, mx * uy, mx * uz, h * ux])
+        Fy = torch.stack([my, my * ux, my * uy + p, my * uz, h * uy])
+        Fz = torch.stack([mz, mz * ux, mz * uy, mz * uz + p, h * uz])
+        return -(deriv(Fx, ikx) + deriv(Fy, iky) + deriv(Fz, ikz))
+
+    def step(U, dt):
+        k1 = rhs(U)
+        k2 = rhs(U + 0.5 * dt * k1)
+        k3 = rhs(U + 0.5 * dt * k2)
+        k4 = rhs(U + dt * k3)
+        Un = U + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+        return torch.fft.ifftn(masked(torch.fft.fftn(Un, dim=dims)), dim=dims).real
+
+    U0 = taylor_green_3d_torch(n, dev, torch.float64)
+    U = step(U0, dt)  # warm-up (FFT plans)
+    torch.cuda.synchronize()
+    U = U0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        U = step(U, dt)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    out = {"workload": f"TGV {n}^3 FP64, the reference's torch backend arithmetic on torch.fft, "
+                       f"{steps} RK4 step(s) after 1 warm-up",
+           "ms_per_step": round(el * 1e3, 2), "mcell_updates_per_s": round(n ** 3 / el / 1e6, 2)}
+    if hip_state is not None:
+        out["rel_l2_vs_hip"] = [float(f"{float(torch.linalg.vector_norm(U[f] - hip_state[f]) / max(float(torch.linalg.vector_norm(hip_state[f])), 1e-300)):.3e}")
+                                for f in range(5)]
+    return out
+
+
 def hybrid_study(nx: int, nys, steps: int, dev):
     """2D Kelvin-Helmholtz on the hybrid grid (Fourier x on [0,1), LGL y on
     [-1/2, 1/2]): shear layers at y = +-1/4 of width 0.05, rho 1 -> 2, p = 2.5,
@@ -233,6 +304,44 @@ def hybrid_study(nx: int, nys, steps: int, dev):
     return {"workload": f"KHI 2D hybrid Fourier-x ({nx}) x LGL-y with walls, {steps} RK4 steps, "
                         f"dt frozen, CFL 0.3",
             **out}
+
+
+def gll_saturation(dev, runs: int = 64, nx: int = 256, nys=(128, 256), reps: int = 20):
+    """The D.U kernel (ph_gll_apply) over a batch of `runs` hybrid-KHI-shaped
+    states at once (4 fields x runs, Ny x nx each): the matrix cores at a size
+    that fills the GPU, as an ensemble of hybrid runs would present it.
+    Useful flops 2 Ny^2 nx per field and run; split modes issue 2 (FP32 data)
+    or 3 (FP64 data) FP32 MFMA products per useful one.  HIP events around
+    `reps` launches on the current stream."""
+    import torch
+    from phrike_amd.basis import LegendreLobattoBasis1D
+
+    out = {}
+    for ny in nys:
+        b = LegendreLobattoBasis1D(ny, 1.0, torch_device=str(dev))
+        F64 = torch.randn(4 * runs, ny, nx, dtype=torch.float64, device=dev)
+        F32 = F64.float()
+        flops = 2.0 * ny * ny * nx * 4 * runs
+        row = {}
+        for name, F, mode, peak in (("f64", F64, "f64", 78.6e12), ("split_f64io", F64, "split", 157.3e12),
+                                    ("split_f32", F32, "split", 157.3e12)):
+            o = torch.empty_like(F)
+            b.apply(F, mode=mode, out=o)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                b.apply(F, mode=mode, out=o)
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) / reps / 1e3
+            issued = flops * (1 if mode == "f64" else (3 if F.dtype == torch.float64 else 2))
+            row[name] = {"avg_us": round(t * 1e6, 1), "useful_TFLOPs": round(flops / t / 1e12, 2),
+                         "mfma_TFLOPs": round(issued / t / 1e12, 2),
+                         "mfma_frac": round(issued / t / peak, 3)}
+        out[f"ny{ny}"] = row
+    return {"workload": f"D.U over {runs} runs x 4 fields, Ny x {nx}, {reps} launches; peak FP64 "
+                        "matrix 78.6, FP32 matrix 157.3 TFLOP/s", **out}
 
 
 def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0):
@@ -352,6 +461,8 @@ def main():
                     help="second size of the FP32-vs-FP64 study (0: none)")
     ap.add_argument("--precision-steps", type=int, default=200)
     ap.add_argument("--hybrid-steps", type=int, default=1000)
+    ap.add_argument("--torch-steps", type=int, default=2,
+                    help="steps of the reference's torch backend arithmetic timed beside (0: none)")
     ap.add_argument("--ensemble-window", type=float, default=3.0)
     ap.add_argument("--ensemble-rate", type=float, default=20.0)
     ap.add_argument("--ensemble-budget-gb", type=float, default=48.0)
@@ -473,10 +584,29 @@ def main():
     sums = s.conserved_sums(U)
     del out
     extras = {}
+    if not args.no_extras and args.torch_steps > 0:
+        print(f"[bench] the reference's torch backend on torch.fft: TGV {n}^3 FP64 x {args.torch_steps}",
+              file=sys.stderr, flush=True)
+        del U, s, g
+        torch.cuda.empty_cache()
+        g2 = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                    filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip", precision="double")
+        s2 = SpectralSolver3D(g2, EulerEquations3D(1.4), scheme="rk4", cfl=0.4)
+        V = taylor_green_3d_torch(n, dev, torch.float64)
+        dt2 = s2.compute_dt(V)
+        for _ in range(args.torch_steps):
+            V = s2.step(V, dt2)
+        del g2, s2
+        torch.cuda.empty_cache()
+        extras["torch_backend"] = torch_backend_reference(n, args.torch_steps, dev, dt2, V)
+        if args.precision == "double":
+            extras["torch_backend"]["hip_speedup"] = round(extras["torch_backend"]["ms_per_step"] / ms_per_step, 2)
+        del V
+        torch.cuda.empty_cache()
     if not args.no_extras:
         print(f"[bench] precision study: turbulence {args.precision_size}^3 x {args.precision_steps}",
               file=sys.stderr, flush=True)
-        del U, s, g
+        U = s = g = None
         torch.cuda.empty_cache()
         extras["fp32_vs_fp64"] = precision_study(args.precision_size, args.precision_steps, dev)
         torch.cuda.empty_cache()
@@ -487,6 +617,7 @@ def main():
         if args.hybrid_steps > 0:
             print(f"[bench] hybrid KHI: {args.hybrid_steps} steps", file=sys.stderr, flush=True)
             extras["hybrid_khi"] = hybrid_study(256, (128, 256), args.hybrid_steps, dev)
+            extras["gll_batched"] = gll_saturation(dev)
         if args.ensemble_window > 0:
             print(f"[bench] ensemble: {args.ensemble_rate} req/s over {args.ensemble_window} s",
                   file=sys.stderr, flush=True)
