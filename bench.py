@@ -306,6 +306,60 @@ def hybrid_study(nx: int, nys, steps: int, dev):
             **out}
 
 
+def mhd_study(n: int, steps: int, dev):
+    """3D ideal MHD (SpectralSolverMHD3D, reference defaults: projection after
+    every substage, filter p=16 alpha=36, gamma 5/3, CFL 0.3) on a band-limited
+    random state n^3 with a non-solenoidal B (seeded; the first projection
+    removes its longitudinal part): time per step after one warm-up step in
+    FP64 and FP32, per-kernel times, max |div B| and relative mass / energy
+    drift (FP64 sums)."""
+    import torch
+    from phrike_amd import _lib
+    from phrike_amd.grid import Grid3D
+    from phrike_amd.initial_conditions import smooth_random_state_3d
+    from phrike_amd.mhd import MHDEquations3D, SpectralSolverMHD3D
+
+    H = smooth_random_state_3d(n, n, n, seed=31, kmax=6)
+    Bn = smooth_random_state_3d(n, n, n, seed=32, kmax=5)
+    B = np.stack([0.3 * Bn[1] / Bn[0], 0.3 * Bn[2] / Bn[0], 1.0 + 0.3 * Bn[3] / Bn[0]])
+    U0 = np.concatenate([H, B])
+    U0[4] = H[4] + 0.5 * (B ** 2).sum(axis=0)
+    del H, Bn, B
+    out = {"workload": f"ideal MHD {n}^3, band-limited random state (|m| <= 6), B0 = (0, 0, 1) + 30% "
+                       f"random, {steps} RK4 steps, projection after every substage"}
+    for prec in ("double", "single"):
+        g = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                   filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip", precision=prec)
+        g.mhd_config = {}
+        s = SpectralSolverMHD3D(g, MHDEquations3D(5.0 / 3.0), cfl=0.3)
+        U = s._project_B(torch.as_tensor(U0, device=dev).to(g.dtype))
+        dt = s.compute_dt(U)
+        c0 = s.conserved_quantities(U)
+        buf = torch.empty_like(U)
+        s.step_into(U, buf, dt)
+        torch.cuda.synchronize()
+        _lib.timing_collect()
+        _lib.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step_into(U, buf, dt)
+            U, buf = buf, U
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        _lib.timing_enable(False)
+        kt = _lib.timing_collect()
+        c1 = s.conserved_quantities(U)
+        out[prec] = {"ms_per_step": round(el * 1e3, 3), "mcell_updates_per_s": round(n ** 3 / el / 1e6, 1),
+                     "max_div_b": float(f"{s.max_div_b(U):.3e}"),
+                     "mass_drift": float(f"{(c1['mass'] - c0['mass']) / c0['mass']:.3e}"),
+                     "energy_drift": float(f"{(c1['energy'] - c0['energy']) / c0['energy']:.3e}"),
+                     "kernels_ms_per_step": {k: round(ms / steps, 3) for k, (cnt, ms) in sorted(kt.items())}}
+        del g, s, U, buf
+        torch.cuda.empty_cache()
+    out["speedup_f32"] = round(out["double"]["ms_per_step"] / out["single"]["ms_per_step"], 3)
+    return out
+
+
 def gll_saturation(dev, runs: int = 64, nx: int = 256, nys=(128, 256), reps: int = 20):
     """The D.U kernel (ph_gll_apply) over a batch of `runs` hybrid-KHI-shaped
     states at once (4 fields x runs, Ny x nx each): the matrix cores at a size
@@ -461,6 +515,8 @@ def main():
                     help="second size of the FP32-vs-FP64 study (0: none)")
     ap.add_argument("--precision-steps", type=int, default=200)
     ap.add_argument("--hybrid-steps", type=int, default=1000)
+    ap.add_argument("--mhd-size", type=int, default=256, help="ideal-MHD study size (0: none)")
+    ap.add_argument("--mhd-steps", type=int, default=10)
     ap.add_argument("--torch-steps", type=int, default=2,
                     help="steps of the reference's torch backend arithmetic timed beside (0: none)")
     ap.add_argument("--ensemble-window", type=float, default=3.0)
@@ -618,6 +674,10 @@ def main():
             print(f"[bench] hybrid KHI: {args.hybrid_steps} steps", file=sys.stderr, flush=True)
             extras["hybrid_khi"] = hybrid_study(256, (128, 256), args.hybrid_steps, dev)
             extras["gll_batched"] = gll_saturation(dev)
+        if args.mhd_size > 0:
+            print(f"[bench] ideal MHD {args.mhd_size}^3", file=sys.stderr, flush=True)
+            extras["mhd3d"] = mhd_study(args.mhd_size, args.mhd_steps, dev)
+            torch.cuda.empty_cache()
         if args.ensemble_window > 0:
             print(f"[bench] ensemble: {args.ensemble_rate} req/s over {args.ensemble_window} s",
                   file=sys.stderr, flush=True)

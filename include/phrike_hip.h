@@ -42,6 +42,7 @@ extern "C" {
 typedef struct ph_plan3d_s* ph_plan3d_t;
 typedef struct ph_plan2d_s* ph_plan2d_t;
 typedef struct ph_planh_s* ph_planh_t;
+typedef struct ph_planmhd3d_s* ph_planmhd3d_t;
 
 /* Library ABI version (no reference counterpart). */
 int ph_abi_version(void);
@@ -230,6 +231,51 @@ int ph_step2dh(ph_planh_t plan, int nbatch, const void* U, void* out, double dt,
                const int* active_dev, int apply_filter, void* ws, void* stream);
 int ph_max_wave_speed2dh(ph_planh_t plan, int nbatch, const void* U, double* max_dev, double* dt_dev,
                          double cfl, void* ws, void* stream);
+
+/* ------------------------------------------------------------------------
+ * 3D ideal MHD, Fourier in x, y, z.  State U is (8, Nz, Ny, Nx): rho, rho ux,
+ * rho uy, rho uz, E, Bx, By, Bz.  Replaces SpectralSolverMHD3D with the
+ * reference defaults (resistivity and viscosity 0, divergence_clean
+ * "projection", projection after every substage, filter interval 1;
+ * phrike/solver.py:840-1050, 1173), MHDEquations3D (phrike/equations.py:
+ * 911-1100) and Grid3D.project_solenoidal (phrike/grid.py:1648-1684).  Same
+ * grid arguments as ph_plan3d_create.
+ * ------------------------------------------------------------------------ */
+int ph_planmhd3d_create(ph_planmhd3d_t* plan, int nx, int ny, int nz, double lx, double ly, double lz,
+                        int dtype, int dealias, int filter_enabled, int filter_p, double filter_alpha,
+                        double gamma);
+int ph_planmhd3d_destroy(ph_planmhd3d_t plan);
+int ph_planmhd3d_workspace_bytes(ph_planmhd3d_t plan, size_t* bytes);
+
+/* rhs = -(dx Fx + dy Fy + dz Fz) with the 8-component ideal-MHD fluxes.
+ * Replaces SpectralSolverMHD._compute_rhs / _flux_divergence
+ * (phrike/solver.py:932-954) and MHDEquations3D.flux (equations.py:949-1039). */
+int ph_rhs_mhd3d(ph_planmhd3d_t plan, const void* U, void* rhs, void* ws, void* stream);
+
+/* In place: B <- Helmholtz projection of B (Nyquist planes zeroed, every mode
+ * kept), E recomputed from the projected B at the thermal pressure of the
+ * unprojected state.  Replaces SpectralSolverMHD._project_B
+ * (phrike/solver.py:966-986) with Grid3D.project_solenoidal. */
+int ph_project_mhd3d(ph_planmhd3d_t plan, void* U, void* ws, void* stream);
+
+/* One RK4 step, each stage state clamped (rho >= 1e-10, p >= 1e-12 through
+ * prim -> cons) and projected, then the spectral filter and the projection:
+ * out = proj(filter(proj(clamp(RK4(U, dt))))).  U and out must not overlap.
+ * Replaces SpectralSolverMHD.step/_fixed_step/_rk_substages/_clamp
+ * (phrike/solver.py:956-1006, 1037-1050). */
+int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void* ws, void* stream);
+
+/* max(|ux|,|uy|,|uz|) + sqrt(max(a^2 + b^2, 0)) over the grid to max_dev[0]
+ * and/or dt_dev[0] = cfl*min(dx,dy,dz)/max (1e-6 if max <= 0).  Replaces
+ * MHDEquations3D.max_wave_speed (phrike/equations.py:1041-1061) and
+ * SpectralSolverMHD.compute_dt without diffusion (phrike/solver.py:1018-1035). */
+int ph_max_wave_speed_mhd3d(ph_planmhd3d_t plan, const void* U, double* max_dev, double* dt_dev, double cfl,
+                            void* ws, void* stream);
+
+/* FP64 sums to out7_dev[0..6]: mass, momentum x/y/z, energy, magnetic
+ * energy, cross helicity.  Replaces MHDEquations3D.conserved_quantities
+ * (phrike/equations.py:1083-1100). */
+int ph_conserved_sums_mhd3d(ph_planmhd3d_t plan, const void* U, double* out7_dev, void* ws, void* stream);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,15 @@ _SIGNATURES = {
     "ph_runs_advance": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_gll_apply": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, c_void_p]),
+    "ph_planmhd3d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_double, c_double,
+                                    c_int, c_int, c_int, c_int, c_double, c_double]),
+    "ph_planmhd3d_destroy": (c_int, [c_void_p]),
+    "ph_planmhd3d_workspace_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "ph_rhs_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_project_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_step_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "ph_max_wave_speed_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "ph_conserved_sums_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_planh_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
                                 c_int, c_int, c_double, c_double, c_int, c_void_p, c_void_p, c_int, c_int]),
     "ph_planh_destroy": (c_int, [c_void_p]),

@@ -17,7 +17,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphrike_hip.so")
-SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "gll.hip", "hybrid.hip"]
+SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "gll.hip", "hybrid.hip", "mhd.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PH_OFFLOAD_ARCH", "gfx950")

@@ -279,7 +279,7 @@ __device__ __forceinline__ void x_split_store(const cx<T>* buf, cx<T>* dst, long
 // ----------------------------------------------------------------------------
 // x forward
 // ----------------------------------------------------------------------------
-enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2 };
+enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2, XF_MHD3 = 3 };
 
 template <typename T>
 struct XFwdArgs {
@@ -301,6 +301,7 @@ template <int MODE> struct XFwdShape;
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
+template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
 
 // Fluxes (or copies) of the conserved values q (both rows of the thread's x
 // positions) -> forward x transform of two rows at once -> kept modes to
@@ -312,9 +313,31 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
   using G = XGeo<T, NX, DA>;
   using S = XFwdShape<MODE>;
   constexpr int LX = S::LX, NC = S::NC, NT = G::NT, XJ = G::XJ;
-  // primitive recovery (phrike/equations.py:104-117, 150-164)
-  T pr[XJ][2][6];
-  if constexpr (MODE != XF_COPY) {
+  // primitive recovery (phrike/equations.py:104-117, 150-164; MHD 922-933)
+  constexpr int NPR = MODE == XF_MHD3 ? 11 : 6;
+  T pr[XJ][2][NPR];
+  if constexpr (MODE == XF_MHD3) {
+    // rho ux uy uz p* Bx By Bz E uB (E + p*)  (phrike/equations.py:922-933, 949-1000)
+#pragma unroll
+    for (int j = 0; j < XJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const T* u = q[j][r];
+        const T rho = u[0];
+        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
+        const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
+        const T Bx = u[5], By = u[6], Bz = u[7];
+        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+        const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+        const T p = (a.gamma - T(1)) * ((u[4] - kin) - mag);
+        const T ps = p + mag;
+        T* w = pr[j][r];
+        w[0] = rho; w[1] = ux; w[2] = uy; w[3] = uz; w[4] = ps;
+        w[5] = Bx; w[6] = By; w[7] = Bz; w[8] = u[4];
+        w[9] = (ux * Bx + uy * By) + uz * Bz;
+        w[10] = u[4] + ps;
+      }
+  } else if constexpr (MODE != XF_COPY) {
 #pragma unroll
     for (int j = 0; j < XJ; ++j)
 #pragma unroll
@@ -354,6 +377,18 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
           for (int r = 0; r < 2; ++r) {
             if constexpr (MODE == XF_COPY) {
               v[r] = q[j][r][l];
+            } else if constexpr (MODE == XF_MHD3) {
+              // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000)
+              const T* w = pr[j][r];
+              const T ug = w[1 + g], Bg = w[5 + g];
+              const T mg = w[0] * ug;
+              if (l == 0) v[r] = mg;
+              else if (l <= 3) {
+                v[r] = mg * w[l];
+                if (l == g + 1) v[r] = v[r] + w[4];
+                v[r] = v[r] - Bg * w[4 + l];
+              } else if (l == 4) v[r] = w[10] * ug - Bg * w[9];
+              else v[r] = (l - 5 == g) ? T(0) : ug * w[l] - w[l - 4] * Bg;
             } else {
               // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
               const T* w = pr[j][r];
@@ -538,7 +573,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
 
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
-    if constexpr (MODE == CM_FWD_DIV3) return (k % 3) * 5 + k / 3;
+    if constexpr (MODE == CM_FWD_DIV3) return (k % 3) * nc + k / 3;
     else if constexpr (MODE == CM_FWDINV_DIV) return (k % 2) * nc + k / 2;
     else return k;
   };
@@ -613,7 +648,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           oc[(long long)j * ols + l] = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
         }
       });
-      cx<T>* o2 = out + (long long)(5 + c) * a.out_fs + c0;
+      cx<T>* o2 = out + (long long)(nc + c) * a.out_fs + c0;
       fwd(FWD{});
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
@@ -664,7 +699,9 @@ enum XInvMode {
   XI_RHS = 1,      // out = -value
   XI_STAGE3 = 2,   // 3D RK4 stage combine
   XI_STAGE2 = 3,   // 2D RK4 stage combine with positivity clamp
-  XI_ADD = 4       // out += value
+  XI_ADD = 4,      // out += value
+  XI_PROJE = 5     // MHD: value = projected B (3 fields) -> state fields 5-7,
+                   // E recomputed at the thermal pressure of the unprojected state
 };
 
 template <typename T>
@@ -752,6 +789,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
   const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
+  static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
   XSpec<T, NX, DA, NC> sp;
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
@@ -785,6 +823,20 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
 #pragma unroll
         for (int c = 0; c < NC; ++c)
           if (c < nfl) a.out[c * a.fs + o] = a.out[c * a.fs + o] + v[r][c];
+      } else if constexpr (MODE == XI_PROJE) {
+        // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933
+        T* u = a.out;
+        const T rho = u[o];
+        const T ux = u[a.fs + o] / rho, uy = u[2 * a.fs + o] / rho, uz = u[3 * a.fs + o] / rho;
+        const T b0 = u[5 * a.fs + o], b1 = u[6 * a.fs + o], b2 = u[7 * a.fs + o];
+        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+        const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
+        const T p = (a.gamma - T(1)) * ((u[4 * a.fs + o] - kin) - mag);
+        const T Bx = v[r][0], By = v[r][1], Bz = v[r][2];
+        u[4 * a.fs + o] = (p / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+        u[5 * a.fs + o] = Bx;
+        u[6 * a.fs + o] = By;
+        u[7 * a.fs + o] = Bz;
       } else {
         T un[NC], acc[NC], u[NC];
         rk_inputs<T, NC>(a, o, un, acc);

@@ -277,8 +277,10 @@ struct Plan3D : PlanBase {
   size_t offA, offB, offUs, offPart, wsBytes;
   static constexpr int NPART = 1024;
 
+  // na / nb: fields of the spectral buffers A (x pass output) and B (y pass
+  // output): 15 / 10 for the Euler RHS, 24 / 16 for ideal MHD
   Plan3D(int nx_, int ny_, int nz_, double lx_, double ly_, double lz_, bool dealias, bool filt,
-         int p, double alpha, double gamma_)
+         int p, double alpha, double gamma_, int na = 15, int nb = 10)
       : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_) {
     dtype = sizeof(T) == 8 ? 0 : 1;
     if (!supported_len(nx) || !supported_len(ny) || !supported_len(nz))
@@ -309,8 +311,8 @@ struct Plan3D : PlanBase {
       hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncol, hipMemcpyHostToDevice), "copy column sigma-1");
     }
     const size_t cs = sizeof(cx<T>);
-    const size_t szA = 15ull * nz * ny * mxp * cs;
-    const size_t szB = 10ull * nz * ay.nr * mxp * cs;
+    const size_t szA = (size_t)na * nz * ny * mxp * cs;
+    const size_t szB = (size_t)nb * nz * ay.nr * mxp * cs;
     auto al = [](size_t v) { return (v + 255) / 256 * 256; };
     offA = 0;
     offB = al(offA + szA);
@@ -373,8 +375,9 @@ struct Plan3D : PlanBase {
     a.sig = dx.sig;
     a.kk = dx.k;
     a.gamma = T(gamma);
-    KScope ks(mode == XF_FLUX3 ? "xfwd_flux" : "xfwd_copy", st);
+    KScope ks(mode == XF_FLUX3 ? "xfwd_flux" : (mode == XF_MHD3 ? "xfwd_mhd" : "xfwd_copy"), st);
     if (mode == XF_FLUX3) launch_xfwd<T, XF_FLUX3>(nx, ny, nz, a, st);
+    else if (mode == XF_MHD3) launch_xfwd<T, XF_MHD3>(nx, ny, nz, a, st);
     else launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
     check_launch("k_xfwd");
   }
@@ -423,6 +426,30 @@ struct Plan3D : PlanBase {
     if (div) launch_col<CE, CM_FWDINV_DIV>(nz, ay.nr, a, st);
     else launch_col<CE, CM_FWDINV_COPY>(nz, ay.nr, a, st);
     check_launch("k_col(z)");
+  }
+  // forward (fwd) or inverse transform along z of nc fields of B, in place;
+  // only for a plan that keeps every z mode (in place needs compact = natural)
+  void zline(bool fwd, int nc, void* ws, hipStream_t st) const {
+    if (az.nr != nz) throw Error(-9, "zline: the plan must keep every z mode");
+    ColArgs<CE> a{};
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
+    a.out = reinterpret_cast<cx<CE>*>(B(ws));
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = (mx + PW - 1) / PW;
+    a.nc = nc;
+    a.keep = az.keep();
+    a.tw = dz.tw;
+    a.sig = dz.sig;
+    a.kk = dz.k;
+    KScope ks(fwd ? "z_fwd" : "z_inv", st);
+    if (fwd) launch_col<CE, CM_FWD_COPY>(nz, ay.nr, a, st);
+    else launch_col<CE, CM_INV>(nz, ay.nr, a, st);
+    check_launch("k_col(z line)");
   }
   // filter correction along z: kept modes times sx sy sz - 1 (after unit x
   // and y passes), in place in B

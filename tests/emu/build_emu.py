@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "phrike_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libphrike_emu.so")
-SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "hybrid.hip", "emu_stubs.cpp"]
+SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "hybrid.hip", "mhd.hip", "emu_stubs.cpp"]
 
 
 def build(asan: bool = False) -> str:

@@ -133,3 +133,52 @@ class EmuHybrid:
         out = np.zeros(nb, dtype=np.float64)
         self._chk(self.lib.ph_max_wave_speed2dh(self.h, nb, P(U), None, P(out), cfl, P(self.ws), None))
         return out
+
+
+class EmuMHD3D:
+    """The 3D ideal-MHD plan (ph_planmhd3d_*) on the CPU emulation."""
+
+    def __init__(self, lib, nx, ny, nz, L=(2 * np.pi,) * 3, dtype=np.float64, dealias=True,
+                 filt=(1, 16, 36.0), gamma=5.0 / 3.0):
+        self.lib, self.dtype = lib, dtype
+        self.shape = (nz, ny, nx)
+        h = ctypes.c_void_p()
+        code = 0 if dtype == np.float64 else 1
+        self._chk(lib.ph_planmhd3d_create(ctypes.byref(h), nx, ny, nz, *L, code, int(dealias), *filt, gamma))
+        self.h = h
+        n = ctypes.c_size_t()
+        self._chk(lib.ph_planmhd3d_workspace_bytes(h, ctypes.byref(n)))
+        self.ws = np.zeros(n.value // 8 + 64, dtype=np.float64)
+        self.sc = np.zeros(8, dtype=np.float64)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.ph_last_error().decode())
+
+    def rhs(self, U):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        self._chk(self.lib.ph_rhs_mhd3d(self.h, P(U), P(out), P(self.ws), None))
+        return out
+
+    def project(self, U):
+        V = np.array(U, dtype=self.dtype, copy=True, order="C")
+        self._chk(self.lib.ph_project_mhd3d(self.h, P(V), P(self.ws), None))
+        return V
+
+    def step(self, U, dt):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        out = np.zeros_like(U)
+        self._chk(self.lib.ph_step_mhd3d(self.h, P(U), P(out), dt, P(self.ws), None))
+        return out
+
+    def max_speed_dt(self, U, cfl):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        m, d = np.zeros(1), np.zeros(1)
+        self._chk(self.lib.ph_max_wave_speed_mhd3d(self.h, P(U), P(m), P(d), cfl, P(self.ws), None))
+        return float(m[0]), float(d[0])
+
+    def sums(self, U):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        self._chk(self.lib.ph_conserved_sums_mhd3d(self.h, P(U), P(self.sc), P(self.ws), None))
+        return self.sc[:7].copy()

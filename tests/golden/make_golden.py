@@ -21,6 +21,12 @@ composition the closed-form checks cannot pin are recorded:
 * F6  2D KHI (periodic profile) 10 steps of SpectralSolver2D.step with the
       per-step compute_dt, clamp and filter, at 64^2 and 32x16, plus a
       filter-cadence case (phrike/solver.py:428-648).
+* F9  3D ideal MHD (SpectralSolverMHD3D with the reference defaults,
+      phrike/solver.py:840-1050, 1173; MHDEquations3D, phrike/equations.py:
+      911-1100; Grid3D.project_solenoidal / divergence, phrike/grid.py:
+      1635-1684): the circularly polarised Alfven wave on 32x8x8 (RHS, 1 and 5
+      steps) and a band-limited random state with a non-solenoidal B on 16^3
+      (fluxes, RHS, projection, CFL speed, 1 and 3 steps, conserved sums).
 
 The outputs are plain ``.npz`` data (no pickles).  numba, pyfftw and finufft
 are absent here, so the reference runs its plain numpy/scipy branches; the
@@ -225,13 +231,74 @@ def f6(grid_m, eq_m, sol_m, out):
     print("F6", flush=True)
 
 
+def _alfven_ic(e, X, rho0=1.0, p0=0.1, B0=1.0, b1=0.1, m=1, Lx=1.0):
+    # phrike/problems/problem_list/alfven3d.py:24-47 (direction +1, t = 0)
+    k = 2.0 * np.pi * m / Lx
+    By, Bz = b1 * np.cos(k * X), b1 * np.sin(k * X)
+    inv = 1.0 / np.sqrt(rho0)
+    return e.conservative(np.full_like(X, rho0), np.zeros_like(X), -By * inv, -Bz * inv,
+                          np.full_like(X, p0), np.full_like(X, B0), By, Bz)
+
+
+def f9(grid_m, eq_m, sol_m, out):
+    res = {}
+    mhd = {"resistivity": 0.0, "viscosity": 0.0, "divergence_clean": "projection",
+           "project_each_substage": True}
+    fp8 = {"enabled": True, "p": 8, "alpha": 36.0}
+    g = grid_m.Grid3D(32, 8, 8, 1.0, 1.0, 1.0, dealias=True, filter_params=dict(fp8), fft_workers=1)
+    g.mhd_config = dict(mhd)
+    e = eq_m.MHDEquations3D(gamma=5.0 / 3.0)
+    s = sol_m.SpectralSolverMHD3D(g, e, scheme="rk4", cfl=0.3)
+    X, _, _ = g.xyz_mesh()
+    U0 = _alfven_ic(e, X)
+    dt = s.compute_dt(U0)
+    res["alf_U0"], res["alf_dt"] = U0, np.float64(dt)
+    res["alf_rhs"] = s._compute_rhs(U0)
+    U = U0
+    for k in range(5):
+        U = s.step(U, dt)
+        if k == 0:
+            res["alf_U1"] = U
+    res["alf_U5"] = U
+    # band-limited random state, 16^3 on [0, 2 pi)^3, B with a longitudinal part
+    g = grid_m.Grid3D(16, 16, 16, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
+                      filter_params=dict(FILT), fft_workers=1)
+    g.mhd_config = dict(mhd)
+    s = sol_m.SpectralSolverMHD3D(g, e, scheme="rk4", cfl=0.3)
+    H = ics.smooth_random_state_3d(16, 16, 16, seed=21, kmax=4)
+    Bn = ics.smooth_random_state_3d(16, 16, 16, seed=22, kmax=3)
+    B = np.stack([0.3 * (Bn[1] / Bn[0]), 0.3 * (Bn[2] / Bn[0]), 1.0 + 0.3 * (Bn[3] / Bn[0])])
+    U = np.concatenate([H, B])
+    U[4] = H[4] + 0.5 * (B[0] ** 2 + B[1] ** 2 + B[2] ** 2)
+    res["rnd_U"] = U
+    Fx, Fy, Fz = e.flux(U)
+    res["rnd_Fx"], res["rnd_Fy"], res["rnd_Fz"] = Fx, Fy, Fz
+    res["rnd_rhs"] = s._compute_rhs(U)
+    res["rnd_proj"] = np.array(g.project_solenoidal(U[5], U[6], U[7]))
+    res["rnd_div"] = g.divergence(U[5], U[6], U[7])
+    res["rnd_maxc"] = np.float64(e.max_wave_speed(U))
+    dt = s.compute_dt(U)
+    res["rnd_dt"] = np.float64(dt)
+    cq = e.conserved_quantities(U)
+    res["rnd_cons_keys"] = np.array(list(cq.keys()))
+    res["rnd_cons"] = np.array(list(cq.values()))
+    V = U
+    for k in range(3):
+        V = s.step(V, dt)
+        if k == 0:
+            res["rnd_U1"] = V
+    res["rnd_U3"] = V
+    np.savez_compressed(os.path.join(out, "mhd3d.npz"), **res, **_meta())
+    print("F9", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-long", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     grid_m, eq_m, sol_m, leg_m = _load_reference()
-    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6"}
+    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6", "f9"}
     if "f3" in only:
         f3(grid_m, eq_m, sol_m, HERE)
     if "f5" in only:
@@ -242,6 +309,8 @@ def main():
         f1_f2(grid_m, eq_m, sol_m, HERE, a.skip_long)
     if "f8" in only:
         f8(grid_m, eq_m, sol_m, HERE)
+    if "f9" in only:
+        f9(grid_m, eq_m, sol_m, HERE)
 
 
 if __name__ == "__main__":
