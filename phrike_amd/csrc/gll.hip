@@ -24,6 +24,9 @@
 //
 // Each lane keeps the operands of up to KC k-steps in registers and issues
 // their loads together, so one global-load latency is paid per KC steps.
+// Large batches (ny, nx multiples of 64) use a block-tiled form that stages
+// operands in LDS and reuses each fragment for two MFMAs; both forms walk k in
+// the same order, so they give identical results.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -129,6 +132,177 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_split(const float* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block-tiled form (ny and nx multiples of 64): a block of 4 waves computes a
+// 64 x 64 output tile of one batch; each wave owns a 32 x 32 quarter as 2 x 2
+// MFMA tiles, so every operand fragment read from LDS feeds two MFMAs.  K runs
+// in chunks of 16: the chunk's D rows (64 x 16) and F rows (16 x 64) are
+// staged in LDS ([k][i] and [k][x], unit stride across the 16 lanes of a
+// fragment), the next chunk's global loads are issued before the MFMAs of the
+// current one.  Same operand/accumulator layouts as above.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;   // output tile edge
+constexpr int GKC = 16;  // k per staged chunk
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gll_f64_tiled(const double* __restrict__ D, const double* __restrict__ F,
+                                                      const double* __restrict__ add, double* __restrict__ out,
+                                                      int ny, int nx) {
+  __shared__ double As[GKC][GT];
+  __shared__ double Bs[GKC][GT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i0 = blockIdx.y * GT, x0 = blockIdx.x * GT;
+  const long long bo = (long long)blockIdx.z * ny * nx;
+  const int r = lane & 15, q = lane >> 4;
+  const int wi = (wave >> 1) * 32, wx = (wave & 1) * 32;
+  // staging: thread -> (A row ar, k ak..ak+3), (B row bk, x bx..bx+3)
+  const int ar = tid >> 2, ak = (tid & 3) * 4;
+  const int bk = tid >> 4, bx = (tid & 15) * 4;
+  const double* Ag = D + (long long)(i0 + ar) * ny + ak;
+  const double* Bg = F + bo + (long long)bk * nx + x0 + bx;
+  double ra[4], rb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ra[u] = Ag[u];
+    rb[u] = Bg[u];
+  }
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < ny; k0 += GKC) {
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      As[ak + u][ar] = ra[u];
+      Bs[bk][bx + u] = rb[u];
+    }
+    lds_barrier();
+    if (k0 + GKC < ny) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ra[u] = Ag[k0 + GKC + u];
+        rb[u] = Bg[(long long)(k0 + GKC) * nx + u];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < GKC / 4; ++s) {
+      const int kk = 4 * s + q;
+      const double a0 = As[kk][wi + r], a1 = As[kk][wi + 16 + r];
+      const double b0 = Bs[kk][wx + r], b1 = Bs[kk][wx + 16 + r];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long long o = bo + (long long)(i0 + wi + 16 * a + q + 4 * v) * nx + x0 + wx + 16 * c + r;
+        double val = acc[a][c][v];
+        if constexpr (EPI == 1) val = -(add[o] + val);
+        out[o] = val;
+      }
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(256) void k_gll_split_tiled(const float* __restrict__ Dhi, const float* __restrict__ Dlo,
+                                                        const T* __restrict__ F, const T* __restrict__ add,
+                                                        T* __restrict__ out, int ny, int nx, int offset) {
+  constexpr bool LO = sizeof(T) == 8;  // FP64 data: G = Ghi + Glo
+  __shared__ float Ah[GKC][GT], Al[GKC][GT], Bh[GKC][GT];
+  __shared__ float Bl[LO ? GKC : 1][LO ? GT : 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i0 = blockIdx.y * GT, x0 = blockIdx.x * GT;
+  const long long bo = (long long)blockIdx.z * ny * nx;
+  const int r = lane & 15, q = lane >> 4;
+  const int wi = (wave >> 1) * 32, wx = (wave & 1) * 32;
+  const int ar = tid >> 2, ak = (tid & 3) * 4;
+  const int bk = tid >> 4, bx = (tid & 15) * 4;
+  const float* Ahg = Dhi + (long long)(i0 + ar) * ny + ak;
+  const float* Alg = Dlo + (long long)(i0 + ar) * ny + ak;
+  const T* Bg = F + bo + (long long)bk * nx + x0 + bx;
+  T c0[4];  // this thread's columns at row 0 (the offset of D's zero row sums)
+#pragma unroll
+  for (int u = 0; u < 4; ++u) c0[u] = offset ? F[bo + x0 + bx + u] : T(0);
+  float rah[4], ral[4];
+  T rb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    rah[u] = Ahg[u];
+    ral[u] = Alg[u];
+    rb[u] = Bg[u];
+  }
+  f4 c1[2][2], c2[2][2], c3[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      c1[a][c] = f4{0.f, 0.f, 0.f, 0.f};
+      c2[a][c] = c1[a][c];
+      c3[a][c] = c1[a][c];
+    }
+  for (int k0 = 0; k0 < ny; k0 += GKC) {
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      Ah[ak + u][ar] = rah[u];
+      Al[ak + u][ar] = ral[u];
+      const T g = rb[u] - c0[u];
+      const float gh = float(g);
+      Bh[bk][bx + u] = gh;
+      if constexpr (LO) Bl[bk][bx + u] = float(g - double(gh));
+    }
+    lds_barrier();
+    if (k0 + GKC < ny) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        rah[u] = Ahg[k0 + GKC + u];
+        ral[u] = Alg[k0 + GKC + u];
+        rb[u] = Bg[(long long)(k0 + GKC) * nx + u];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < GKC / 4; ++s) {
+      const int kk = 4 * s + q;
+      const float ah0 = Ah[kk][wi + r], ah1 = Ah[kk][wi + 16 + r];
+      const float al0 = Al[kk][wi + r], al1 = Al[kk][wi + 16 + r];
+      const float bh0 = Bh[kk][wx + r], bh1 = Bh[kk][wx + 16 + r];
+      c1[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah0, bh0, c1[0][0], 0, 0, 0);
+      c1[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah0, bh1, c1[0][1], 0, 0, 0);
+      c1[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah1, bh0, c1[1][0], 0, 0, 0);
+      c1[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah1, bh1, c1[1][1], 0, 0, 0);
+      c2[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(al0, bh0, c2[0][0], 0, 0, 0);
+      c2[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(al0, bh1, c2[0][1], 0, 0, 0);
+      c2[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(al1, bh0, c2[1][0], 0, 0, 0);
+      c2[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(al1, bh1, c2[1][1], 0, 0, 0);
+      if constexpr (LO) {
+        const float bl0 = Bl[kk][wx + r], bl1 = Bl[kk][wx + 16 + r];
+        c3[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah0, bl0, c3[0][0], 0, 0, 0);
+        c3[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah0, bl1, c3[0][1], 0, 0, 0);
+        c3[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah1, bl0, c3[1][0], 0, 0, 0);
+        c3[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah1, bl1, c3[1][1], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long long o = bo + (long long)(i0 + wi + 16 * a + 4 * q + v) * nx + x0 + wx + 16 * c + r;
+        double val = double(c1[a][c][v]) + (double(c2[a][c][v]) + double(c3[a][c][v]));
+        if constexpr (EPI == 1) val = -(double(add[o]) + val);
+        out[o] = T(val);
+      }
+}
+
 }  // namespace ph
 
 namespace ph {
@@ -140,9 +314,45 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
   if (!F || !out) throw Error(PH_ERR_ARG, "null buffer");
   const int offset = (mode & PH_GLL_OFFSET) ? 1 : 0;
   mode &= ~PH_GLL_OFFSET;
+  const bool epi = add != nullptr;
+  // the tiled form when it yields at least one block per CU; smaller batches
+  // keep the 16-column blocks of the per-wave form, which spread wider
+  if (ny % GT == 0 && nx % GT == 0 && (long long)(nx / GT) * (ny / GT) * nbatch >= 256) {
+    dim3 grid(nx / GT, ny / GT, nbatch), block(256);
+    switch (mode) {
+      case PH_GLL_F64:
+        if (!D) throw Error(PH_ERR_ARG, "null D");
+        if (epi) k_gll_f64_tiled<1><<<grid, block, 0, st>>>((const double*)D, (const double*)F, (const double*)add,
+                                                            (double*)out, ny, nx);
+        else k_gll_f64_tiled<0><<<grid, block, 0, st>>>((const double*)D, (const double*)F, nullptr, (double*)out,
+                                                        ny, nx);
+        break;
+      case PH_GLL_SPLIT_F32:
+        if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
+        if (epi) k_gll_split_tiled<float, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                                     (const float*)F, (const float*)add,
+                                                                     (float*)out, ny, nx, offset);
+        else k_gll_split_tiled<float, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                                 (const float*)F, nullptr, (float*)out, ny, nx,
+                                                                 offset);
+        break;
+      case PH_GLL_SPLIT_F64IO:
+        if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
+        if (epi) k_gll_split_tiled<double, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                                      (const double*)F, (const double*)add,
+                                                                      (double*)out, ny, nx, offset);
+        else k_gll_split_tiled<double, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
+                                                                  (const double*)F, nullptr, (double*)out, ny,
+                                                                  nx, offset);
+        break;
+      default:
+        throw Error(PH_ERR_ARG, "unknown mode");
+    }
+    check_launch("k_gll(tiled)");
+    return;
+  }
   dim3 block(64 * GLL_WAVES);
   dim3 grid(nx / 16, (ny + 16 * GLL_WAVES - 1) / (16 * GLL_WAVES), nbatch);
-  const bool epi = add != nullptr;
   switch (mode) {
     case PH_GLL_F64:
       if (!D) throw Error(PH_ERR_ARG, "null D");

@@ -49,3 +49,31 @@ def test_gll_reference_tables_on_device(dev):
     got = b.apply(torch.as_tensor(F, device=dev), "f64").cpu().numpy()
     ref = np.einsum("ij,bjx->bix", D, F)
     assert np.abs(got - ref).max() / (np.abs(D).sum(axis=1).max() * np.abs(F).max()) < 1e-13
+
+
+@pytest.mark.parametrize("ny,nx,nb", [(256, 256, 16), (128, 64, 128)])
+def test_gll_tiled_batch_matches_single(dev, ny, nx, nb):
+    """A batch large enough for the LDS-tiled kernel gives, field by field,
+    exactly what single-field launches (the per-wave kernel) give: both walk
+    k in the same order.  Also checked against the oracle."""
+    import torch
+    from oracle import gll_np as gn
+    from phrike_amd.basis import LegendreLobattoBasis1D
+    b = LegendreLobattoBasis1D(ny, 1.0)
+    ob = gn.LGLBasisOracle(ny, 1.0)
+    rng = np.random.default_rng(7 * ny + nb)
+    F = rng.standard_normal((nb, ny, nx)) + 3.0
+    Ft = torch.as_tensor(F, device=dev)
+    A = torch.as_tensor(rng.standard_normal((nb, ny, nx)), device=dev)
+    ref = gn.apply_along_y(ob.D_x, F)
+    scale = np.abs(ob.D_x).sum(axis=1).max() * np.abs(F).max()
+    for mode, X, add in (("f64", Ft, None), ("split", Ft, None), ("split", Ft.float(), None),
+                         ("f64", Ft, A), ("split", Ft.float(), A.float())):
+        whole = b.apply(X, mode, add=add)
+        for i in (0, nb // 2, nb - 1):
+            one = b.apply(X[i:i + 1].contiguous(), mode,
+                          add=None if add is None else add[i:i + 1].contiguous())
+            assert torch.equal(whole[i:i + 1], one), (mode, X.dtype, add is None, i)
+        if add is None:
+            err = np.abs(whole.double().cpu().numpy() - ref).max() / scale
+            assert err < (1e-14 if mode == "f64" else 1e-6), (mode, X.dtype, err)
