@@ -522,6 +522,8 @@ def main():
     ap.add_argument("--ensemble-window", type=float, default=3.0)
     ap.add_argument("--ensemble-rate", type=float, default=20.0)
     ap.add_argument("--ensemble-budget-gb", type=float, default=48.0)
+    ap.add_argument("--gll-only", action="store_true",
+                    help="only run the batched D.U matrix-core study (for profiling the MFMA kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (HBM counter calibration)")
     ap.add_argument("--traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV", "CAL_WRITE_CSV"),
@@ -560,6 +562,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read_per_launch": a.numel() * 8,
                           "bytes_written_per_launch": b.numel() * 8, "launches": 5}))
+        return
+    if args.gll_only:
+        print(json.dumps({"gll_batched": gll_saturation(torch.device("cuda:0"))}))
         return
     from phrike_amd import _lib
     from phrike_amd.equations import EulerEquations3D
