@@ -700,8 +700,10 @@ enum XInvMode {
   XI_STAGE3 = 2,   // 3D RK4 stage combine
   XI_STAGE2 = 3,   // 2D RK4 stage combine with positivity clamp
   XI_ADD = 4,      // out += value
-  XI_PROJE = 5     // MHD: value = projected B (3 fields) -> state fields 5-7,
+  XI_PROJE = 5,    // MHD: value = projected B (3 fields) -> state fields 5-7,
                    // E recomputed at the thermal pressure of the unprojected state
+  XI_MHDSTAGE = 6  // MHD: k = -value; RK4 stage combine in the reference's order
+                   // and the clamp, next state to us, the k sum in out
 };
 
 template <typename T>
@@ -766,6 +768,42 @@ __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC],
   if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
 }
 
+// MHD stage combine at one point (phrike/solver.py:988-1006) with k = -v:
+// stage 1: acc = k; stages 2, 3: acc = acc + 2k; next state U + c k (c = dt/2,
+// dt/2, dt); stage 4: U + dt/6 (acc + k).  Then the clamp rho >= rho_min,
+// p >= p_min through prim -> cons (phrike/solver.py:956-964) into u.
+template <typename T>
+__device__ __forceinline__ void mhd_stage_point(const XInvArgs<T>& a, const T (&v)[8], long long o, double dtd,
+                                                T (&u)[8]) {
+  const T c = T(a.stage == 3 ? dtd : 0.5 * dtd), sixth = T(dtd / 6.0);
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    const long long idx = f * a.fs + o;
+    const T k = -v[f];
+    const T un = a.un[idx];
+    if (a.stage < 4) {
+      const T acc = a.stage == 1 ? k : a.out[idx] + T(2) * k;
+      a.out[idx] = acc;
+      u[f] = un + c * k;
+    } else {
+      u[f] = un + sixth * (a.out[idx] + k);
+    }
+  }
+  const T rho = u[0];
+  const T ux = u[1] / rho, uy = u[2] / rho, uz = u[3] / rho;
+  const T Bx = u[5], By = u[6], Bz = u[7];
+  const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+  const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+  const T p = (a.gamma - T(1)) * ((u[4] - kin) - mag);
+  const T rc = rho < a.rho_min ? a.rho_min : rho;  // NaN propagates like np.maximum
+  const T pc = p < a.p_min ? a.p_min : p;
+  u[0] = rc;
+  u[1] = rc * ux;
+  u[2] = rc * uy;
+  u[3] = rc * uz;
+  u[4] = (pc / (a.gamma - T(1)) + (T(0.5) * rc) * ((ux * ux + uy * uy) + uz * uz)) + mag;
+}
+
 // the RK combine's inputs at one point: U (stages 1-3) and the accumulator
 // (stages 2-4)
 template <typename T, int NC>
@@ -790,6 +828,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   PH_XTW_FILL(a.tw, NX, NT, tid);
   const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
   static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
+  static_assert(MODE != XI_MHDSTAGE || NC == 8, "the MHD stage combine takes the 8 MHD fields");
   XSpec<T, NX, DA, NC> sp;
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
@@ -837,6 +876,11 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
         u[5 * a.fs + o] = Bx;
         u[6 * a.fs + o] = By;
         u[7 * a.fs + o] = Bz;
+      } else if constexpr (MODE == XI_MHDSTAGE) {
+        T u[8];
+        mhd_stage_point<T>(a, v[r], o, dtd, u);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) a.us[c * a.fs + o] = u[c];
       } else {
         T un[NC], acc[NC], u[NC];
         rk_inputs<T, NC>(a, o, un, acc);

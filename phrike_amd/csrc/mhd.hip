@@ -8,8 +8,9 @@
 //   RHS    -(dx Fx + dy Fy + dz Fz): the Euler pencil pipeline with 8 fields
 //          (k_xfwd XF_MHD3: fluxes of the three directions in registers, x
 //          transforms; y pass 24 -> 16; z pass 16 -> 8; y and x inverses).
-//   stage  k_mhd_stage: RK4 combine in the reference's order plus the clamp
-//          rho >= 1e-10, p >= 1e-12 through prim -> cons.
+//   stage  the RHS's x inverse (XI_MHDSTAGE) also does the RK4 combine in
+//          the reference's order and the clamp rho >= 1e-10, p >= 1e-12
+//          through prim -> cons, so k never goes to HBM.
 //   proj   an unpruned plan (every mode kept, no filter): B to spectral space
 //          (x, y, z forward), k_mhd_proj zeroes the Nyquist planes and removes
 //          k (k.B^)/k^2, back along z, y, x; the x inverse (XI_PROJE) writes B
@@ -29,47 +30,6 @@ namespace ph {
 
 constexpr double MHD_RHO_MIN = 1e-10;  // phrike/solver.py:877
 constexpr double MHD_P_MIN = 1e-12;    // phrike/solver.py:878
-
-// RK4 combine of stage s (phrike/solver.py:988-1006) with k = K:
-//   s = 1: acc = k; s = 2, 3: acc = acc + 2k; next state U + c k (c = dt/2,
-//   dt/2, dt); s = 4: U + dt/6 (acc + k) -- then clamp(prim -> cons)
-//   (phrike/solver.py:956-964) into `out`.
-template <typename T>
-__global__ __launch_bounds__(NT) void k_mhd_stage(const T* K, const T* U, T* acc, T* out, long long n,
-                                                 int stage, double dt, T gamma, T rho_min, T p_min) {
-  const T c = T(stage == 3 ? dt : 0.5 * dt), sixth = T(dt / 6.0);
-  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
-    T v[8];
-#pragma unroll
-    for (int f = 0; f < 8; ++f) {
-      const T k = K[f * n + i];
-      const T u = U[f * n + i];
-      if (stage < 4) {
-        const T a = stage == 1 ? k : acc[f * n + i] + T(2) * k;
-        acc[f * n + i] = a;
-        v[f] = u + c * k;
-      } else {
-        v[f] = u + sixth * (acc[f * n + i] + k);
-      }
-    }
-    const T rho = v[0];
-    const T ux = v[1] / rho, uy = v[2] / rho, uz = v[3] / rho;
-    const T Bx = v[5], By = v[6], Bz = v[7];
-    const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-    const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-    const T p = (gamma - T(1)) * ((v[4] - kin) - mag);
-    const T rc = rho < rho_min ? rho_min : rho;  // NaN propagates like np.maximum
-    const T pc = p < p_min ? p_min : p;
-    out[i] = rc;
-    out[n + i] = rc * ux;
-    out[2 * n + i] = rc * uy;
-    out[3 * n + i] = rc * uz;
-    out[4 * n + i] = (pc / (gamma - T(1)) + (T(0.5) * rc) * ((ux * ux + uy * uy) + uz * uz)) + mag;
-    out[5 * n + i] = Bx;
-    out[6 * n + i] = By;
-    out[7 * n + i] = Bz;
-  }
-}
 
 // Leray projection of the three B spectra (all modes kept), in place:
 // Nyquist planes zeroed, B^ -= k (k.B^)/k^2 with k^2 = 1 at k = 0.  Element
@@ -195,7 +155,7 @@ struct PlanMHD3D : PlanBase {
   long long ncell;
   double gamma;
   CE *kxc = nullptr, *nyqc = nullptr, *kyv = nullptr, *kzv = nullptr;
-  size_t offMain, offPrj, offK, offAcc, offUs, offPart, wsBytes;
+  size_t offMain, offPrj, offAcc, offUs, offPart, wsBytes;
 
   PlanMHD3D(int nx_, int ny_, int nz_, double lx, double ly, double lz, bool dealias, bool filt, int p,
             double alpha, double gamma_)
@@ -236,8 +196,7 @@ struct PlanMHD3D : PlanBase {
     const size_t st = sizeof(T) * 8ull * ncell;
     offMain = 0;
     offPrj = al(main.wsBytes);
-    offK = al(offPrj + prj.wsBytes);
-    offAcc = al(offK + st);
+    offAcc = al(offPrj + prj.wsBytes);
     offUs = al(offAcc + st);
     offPart = al(offUs + st);
     wsBytes = al(offPart + sizeof(double) * NPART * 7);
@@ -248,7 +207,6 @@ struct PlanMHD3D : PlanBase {
   }
   void* wsMain(void* ws) const { return (char*)ws + offMain; }
   void* wsPrj(void* ws) const { return (char*)ws + offPrj; }
-  T* K(void* ws) const { return reinterpret_cast<T*>((char*)ws + offK); }
   T* Acc(void* ws) const { return reinterpret_cast<T*>((char*)ws + offAcc); }
   T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
   double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
@@ -294,11 +252,26 @@ struct PlanMHD3D : PlanBase {
     launch_xinv<T, 3, XI_PROJE>(nx, ny, nz, a, st);
     check_launch("k_xinv(mhd projection)");
   }
-  void stage(int s, const T* U, T* out, double dt, void* ws, hipStream_t st) const {
-    KScope ks("mhd_stage", st);
-    hipLaunchKernelGGL((k_mhd_stage<T>), dim3(blocks(ncell) * 4), dim3(NT), 0, st, K(ws), U, Acc(ws), out, ncell,
-                       s, dt, T(gamma), T(MHD_RHO_MIN), T(MHD_P_MIN));
-    check_launch("k_mhd_stage");
+  // RHS of stage state Uin fused with the stage-s combine: the accumulator in
+  // Acc(ws), the next (clamped) state to out (phrike/solver.py:932-964, 988-1006)
+  void rhs_stage(int s, const T* Uin, const T* U, T* out, double dt, void* ws, hipStream_t st) const {
+    void* w = wsMain(ws);
+    main.xfwd(XF_MHD3, Uin, 8, 0xFFu, w, st);
+    main.yfwd(true, 8, 0, w, st);
+    main.zpass(true, 8, 0, w, st);
+    main.yinv(8, w, st);
+    XInvArgs<T> a = main.xinv_args(w);
+    a.nf = 8;
+    a.out = Acc(ws);
+    a.un = U;
+    a.us = out;
+    a.stage = s;
+    a.dt = dt;
+    a.rho_min = T(MHD_RHO_MIN);
+    a.p_min = T(MHD_P_MIN);
+    KScope ks("xinv_mhdstage", st);
+    launch_xinv<T, 8, XI_MHDSTAGE>(nx, ny, nz, a, st);
+    check_launch("k_xinv(mhd stage)");
   }
   // the main plan's spectral filter (mask and sigma) of all 8 fields, in place
   void filter(T* U, void* ws, hipStream_t st) const {
@@ -311,8 +284,8 @@ struct PlanMHD3D : PlanBase {
   void step(const T* U, T* out, double dt, void* ws, hipStream_t st) const {
     T* us = Us(ws);
     for (int s = 1; s <= 4; ++s) {
-      rhs(s == 1 ? U : us, K(ws), ws, st);
-      stage(s, U, s < 4 ? us : out, dt, ws, st);
+      // stages 2, 3 overwrite us, which their forward x pass has read by then
+      rhs_stage(s, s == 1 ? U : us, U, s < 4 ? us : out, dt, ws, st);
       project(s < 4 ? us : out, ws, st);
     }
     filter(out, ws, st);
