@@ -81,9 +81,15 @@ struct Keep {
 
 // Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
 // of the block's 2*NX points, which bounds the per-thread flux and FFT state.
+#ifndef PH_XT_MAX_F64
+#define PH_XT_MAX_F64 1024
+#endif
+#ifndef PH_XT_MAX_F32
+#define PH_XT_MAX_F32 512
+#endif
 template <typename T, int NX>
 struct XThreads {
-  static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, 512) : clampi(NX / 2, 64, 256);
+  static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, PH_XT_MAX_F64) : clampi(NX / 2, 64, PH_XT_MAX_F32);
 };
 
 // LDS line stride (complex values per x position) of the x kernels: by
@@ -380,7 +386,10 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
             } else if constexpr (MODE == XF_MHD3) {
               // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000)
               const T* w = pr[j][r];
-              const T ug = w[1 + g], Bg = w[5 + g];
+              // direction g picked by selects: a register index by the runtime
+              // g would put pr in scratch memory
+              const T ug = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
+              const T Bg = g == 0 ? w[5] : (g == 1 ? w[6] : w[7]);
               const T mg = w[0] * ug;
               if (l == 0) v[r] = mg;
               else if (l <= 3) {
@@ -392,7 +401,7 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
             } else {
               // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
               const T* w = pr[j][r];
-              const T ud = w[1 + g];
+              const T ud = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
               const T md = w[0] * ud;
               if (l == 0) v[r] = md;
               else if (l == NC - 1) v[r] = w[5] * ud;

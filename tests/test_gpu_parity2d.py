@@ -77,3 +77,32 @@ def test_fp32_vs_fp64(dev):
     print("fp32 KHI 64^2 x 10 per-field rel L2:", field_errors(U.double().cpu(), d["sq64_U10"]),
           "group-normalised:", errs, flush=True)
     assert errs.max() < 1e-5
+
+
+@pytest.mark.parametrize("nx,ny", [(1024, 32), (512, 16)])
+def test_long_x_lines_vs_oracle(dev, nx, ny):
+    """x lines of 512 and 1024 points (the widest x kernels: in FP64 one
+    thread per x position of a row pair, in FP32 one per two) against the
+    oracle: two steps of a periodic KHI, two runs in one batch; FP32 bounded
+    against the same oracle."""
+    import torch
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import khi_2d_periodic
+    og = sn.Grid2DOracle(nx, ny, 1.0, 1.0, filter_params=FILT, fft_workers=1)
+    osol = sn.Solver2DOracle(og, sn.Euler2DOracle(1.4), cfl=0.4)
+    V = [khi_2d_periodic(nx, ny, seed=s) for s in (3, 4)]
+    dt = osol.compute_dt(V[0])
+    ref = [osol.step(osol.step(v, dt), dt) for v in V]
+    for prec, tol in (("double", 1e-12), ("single", 1e-5)):
+        g, e, s = _setup(nx, ny, precision=prec, batch=2)
+        U = torch.as_tensor(np.stack(V), device=dev).to(g.dtype)
+        print(f"case {nx}x{ny} {prec}", flush=True)
+        for _ in range(2):
+            U = s.step(U, dt)
+        for k in range(2):
+            if prec == "double":
+                err = field_errors(U[k].cpu(), ref[k]).max()
+            else:
+                err = group_errors(U[k].double().cpu(), ref[k], [(0,), (1, 2), (3,)]).max()
+            print(f"  run {k}: {err:.2e}", flush=True)
+            assert err < tol, (prec, k, err)

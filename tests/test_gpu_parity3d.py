@@ -143,6 +143,32 @@ def test_nonuniform_box_vs_oracle(dev):
     assert _v < 1e-12
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(1024, 8, 16), (512, 16, 8)])
+def test_long_x_lines_3d_vs_oracle(dev, nx, ny, nz):
+    """3D x lines of 1024 and 512 points (the widest x kernels) against the
+    oracle, two steps in FP64; FP32 bounded against the same oracle."""
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import smooth_random_state_3d
+    import torch
+    U0 = smooth_random_state_3d(nx, ny, nz, seed=5, kmax=6)
+    og = sn.Grid3DOracle(nx, ny, nz, 2 * np.pi, 2 * np.pi, 2 * np.pi, filter_params=dict(FILT))
+    os_ = sn.Solver3DOracle(og, sn.Euler3DOracle(1.4), cfl=0.4)
+    dt = os_.compute_dt(U0)
+    Uo = os_.step(os_.step(U0, dt), dt)
+    for prec in ("double", "single"):
+        print(f"case {nx}x{ny}x{nz} {prec}", flush=True)
+        g, e, s = _setup(nx, ny, nz, precision=prec)
+        U = _t(U0, dev, torch.float64 if prec == "double" else torch.float32)
+        U = s.step(s.step(U, dt), dt)
+        if prec == "double":
+            _v = field_errors(U.cpu(), Uo).max()
+            assert _v < 1e-12, _v
+        else:
+            _v = group_errors(U.double().cpu(), Uo, [(0,), (1, 2, 3), (4,)]).max()
+            assert _v < 1e-5, _v
+        print(f"  {_v:.2e}", flush=True)
+
+
 def test_conserved_sums_vs_oracle(dev):
     from oracle import spectral_np as sn
     from phrike_amd.initial_conditions import smooth_random_state_3d
