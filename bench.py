@@ -398,6 +398,33 @@ def gll_saturation(dev, runs: int = 64, nx: int = 256, nys=(128, 256), reps: int
                         "matrix 78.6, FP32 matrix 157.3 TFLOP/s", **out}
 
 
+def hbm_reference(dev, gib: int = 2, reps: int = 5):
+    """What plain torch kernels reach on this GPU over `gib` GiB buffers: a
+    device copy (bytes read + written), a read-only sum and a write-only fill;
+    HIP events around `reps` launches each.  The achievable rate a mixed
+    read/write pencil kernel can be compared with, beside the 8 TB/s peak."""
+    import torch
+
+    a = torch.ones(gib << 27, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    nbytes = a.numel() * 8
+    out = {}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 2 * nbytes), ("read", lambda: a.sum(), nbytes),
+                            ("write", lambda: b.fill_(2.0), nbytes)):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[f"{name}_GBs"] = round(moved * reps / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"workload": f"torch copy_/sum/fill_ over {gib} GiB FP64 buffers, {reps} launches each", **out}
+
+
 def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0):
     """Poisson stream (mean `rate` req/s over `window` s) of 2D Fourier KHI runs,
     sizes {128, 256, 512, 1024}^2 with weights 0.4/0.3/0.2/0.1, own seed each,
@@ -475,8 +502,10 @@ def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, n, prec, c
     es = 8 if prec == "double" else 4
     per, _ = byte_model(n)
     alg = {}
+    launches = step_launches(prec)
     for tag, units in per.items():
-        alg.setdefault(kernel_symbol(tag, n, prec), []).extend([units * es * n ** 3] * step_launches(prec).get(tag, 1))
+        if launches.get(tag, 0) > 0:  # tags of this precision's step only
+            alg.setdefault(kernel_symbol(tag, n, prec), []).extend([units * es * n ** 3] * launches[tag])
     acc = {}
     for did, (name, v) in fe.items():
         acc.setdefault(name, [0, 0.0, 0.0])
@@ -645,6 +674,10 @@ def main():
     sums = s.conserved_sums(U)
     del out
     extras = {}
+    if not args.no_extras:
+        extras["hbm_reference"] = hbm_reference(dev)
+        if roof is not None:
+            roof["frac_of_copy"] = round(roof["achieved"] / extras["hbm_reference"]["copy_GBs"], 4)
     if not args.no_extras and args.torch_steps > 0:
         print(f"[bench] the reference's torch backend on torch.fft: TGV {n}^3 FP64 x {args.torch_steps}",
               file=sys.stderr, flush=True)
