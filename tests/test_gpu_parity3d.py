@@ -123,8 +123,11 @@ def test_tgv256_vs_reference(dev):
             idx = np.r_[0:9, -8:0]
             low = low[:, idx][:, :, idx][:, :, :, :9]
             lerr = field_errors(low.view(np.float64), d[f"low8_{k}"].view(np.float64))
-            print(f"256^3 step {k}: sub8 {errs.max():.2e} lowmodes {lerr.max():.2e}", flush=True)
-            assert errs.max() <= 1e-11 and lerr.max() <= 1e-11
+            sums = s.conserved_sums(U)
+            cerr = np.abs(sums[[0, 4]] / d[f"conserved_{k}"][[0, 4]] - 1).max()
+            print(f"256^3 step {k}: sub8 {errs.max():.2e} lowmodes {lerr.max():.2e} "
+                  f"mass/energy {cerr:.1e}", flush=True)
+            assert errs.max() <= 1e-11 and lerr.max() <= 1e-11 and cerr <= 1e-12
 
 
 def test_nonuniform_box_vs_oracle(dev):
@@ -143,10 +146,11 @@ def test_nonuniform_box_vs_oracle(dev):
     assert _v < 1e-12
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(1024, 8, 16), (512, 16, 8)])
-def test_long_x_lines_3d_vs_oracle(dev, nx, ny, nz):
-    """3D x lines of 1024 and 512 points (the widest x kernels) against the
-    oracle, two steps in FP64; FP32 bounded against the same oracle."""
+@pytest.mark.parametrize("nx,ny,nz", [(1024, 8, 16), (512, 16, 8), (16, 1024, 8), (8, 16, 1024)])
+def test_long_lines_3d_vs_oracle(dev, nx, ny, nz):
+    """3D lines of 1024 points along x, y or z (and 512 along x: the widest x
+    kernels and the longest column transforms) against the oracle, two steps
+    in FP64; FP32 bounded against the same oracle."""
     from oracle import spectral_np as sn
     from phrike_amd.initial_conditions import smooth_random_state_3d
     import torch
@@ -236,3 +240,36 @@ def test_fp32_filter_correction_form_vs_fp64(dev):
     print("fp32 vs fp64, correction form:", err_corr, "direct form:", err_direct, flush=True)
     assert err_corr.max() < 1e-5
     assert err_corr[1:4].max() < 0.5 * err_direct[1:4].max()
+
+
+def test_tgv1024_fp32_matches_512(dev):
+    """The largest grid, 1024^3 in FP32 (spectral buffers of more than 2^31
+    elements): two TGV steps at the dt of 512^3 keep mass to round-off, and
+    the Fourier modes |m| <= 8 agree with the 512^3 run's (the TGV is resolved
+    at both sizes and the filter is ~1 at these modes)."""
+    import torch
+    from phrike_amd.initial_conditions import taylor_green_3d_torch
+    low = {}
+    dt = None
+    for n in (512, 1024):
+        print(f"case TGV {n}^3 fp32", flush=True)
+        g, e, s = _setup(n, n, n, precision="single")
+        U = taylor_green_3d_torch(n, dev, torch.float32)
+        if dt is None:
+            dt = s.compute_dt(U)
+        m0 = float(s.conserved_sums(U)[0])
+        out = torch.empty_like(U)
+        for _ in range(2):
+            s.step_into(U, out, dt)
+            U, out = out, U
+        del out
+        m1 = float(s.conserved_sums(U)[0])
+        assert abs(m1 / m0 - 1) < 1e-6, (n, m0, m1)
+        F = torch.fft.rfftn(U, dim=(-3, -2, -1)) / float(n) ** 3
+        idx = torch.tensor(list(range(9)) + list(range(-8, 0)), device=dev) % n
+        low[n] = F[:, idx][:, :, idx][:, :, :, :9].to(torch.complex128).cpu().numpy()
+        del U, F, g, e, s
+        torch.cuda.empty_cache()
+    d = np.abs(low[1024] - low[512]).max() / np.abs(low[512]).max()
+    print(f"1024^3 vs 512^3 low modes after 2 steps: {d:.2e}", flush=True)
+    assert d < 1e-5
