@@ -44,7 +44,9 @@ typedef struct ph_plan2d_s* ph_plan2d_t;
 typedef struct ph_planh_s* ph_planh_t;
 typedef struct ph_planmhd3d_s* ph_planmhd3d_t;
 
-/* Library ABI version (no reference counterpart). */
+/* Library ABI version (no reference counterpart): PH_ABI_VERSION of the
+ * header the library was built with. */
+#define PH_ABI_VERSION 1
 int ph_abi_version(void);
 
 /* Per-kernel timing (no reference counterpart; measurement only).  While

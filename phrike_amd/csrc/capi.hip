@@ -73,7 +73,7 @@ void set_last_error(const char* msg) { g_err = msg; }
 
 extern "C" {
 
-int ph_abi_version(void) { return 1; }
+int ph_abi_version(void) { return PH_ABI_VERSION; }
 
 int ph_timing_enable(int on) {
   return guard([&] {
