@@ -24,7 +24,10 @@
 //
 // Per stage: k_hx (fluxes, x derivative of Fx with both rows of a pair in one
 // complex transform; Fy written as is) -> D.Fy with the epilogue
-// rhs = -(dFx + D.Fy) -> k_hrk (RK4 combine + clamp, pointwise).
+// rhs = -(dFx + D.Fy).  The RK4 combine + clamp of stage s runs inside the
+// k_hx of stage s+1 (and the stage-4 combine inside the filter's k_hx), so
+// the stage states never go to HBM; k_hrk does it as a separate pass when the
+// step has no filter.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -52,6 +55,14 @@ struct HxArgs {
   T gamma;
   const int* active;
   int bc;
+  // combine mode: the input is the RK4 combine of stage `stage` from its
+  // derivative rhs, the step's base state un and the accumulator acc
+  const T* rhs;
+  const T* un;
+  T* acc;
+  int stage;
+  double dt;
+  const double* dt_dev;  // per run, or null
 };
 
 // wall rule on a boundary row: my = 0, rho and E from the adjacent row
@@ -63,7 +74,34 @@ __device__ __forceinline__ void wall_row(T (&q)[4], const T* nb, long long fs) {
   q[3] = nb[3 * fs];
 }
 
-template <typename T, int NX, int MODE>
+// RK4 combine of stage a.stage at point o (row y) with k = rhs, clamp
+// (phrike/solver.py:467-513): stages 1-3 give the next stage state and, with
+// write_acc, update the accumulator (k1, then + 2 k2, + 2 k3); stage 4 gives
+// U + dt/6 (acc + k4).  The base state takes the wall rule on boundary rows.
+template <typename T>
+__device__ __forceinline__ void hyb_combine(const HxArgs<T>& a, long long o, int y, int nx, double d,
+                                            bool write_acc, T (&u)[4]) {
+  const T half = T(0.5 * d), full = T(d), sixth = T(d / 6.0);
+  T b[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) b[c] = a.un[c * a.fs + o];
+  if (a.bc && (y == 0 || y == a.ny - 1)) wall_row(b, a.un + o + (y == 0 ? nx : -nx), a.fs);
+  if (a.stage < 4) {
+    const T coef = a.stage == 3 ? full : half;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const T k = a.rhs[c * a.fs + o];
+      if (write_acc) a.acc[c * a.fs + o] = a.stage == 1 ? k : a.acc[c * a.fs + o] + T(2) * k;
+      u[c] = b[c] + coef * k;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) u[c] = b[c] + sixth * (a.acc[c * a.fs + o] + a.rhs[c * a.fs + o]);
+  }
+  clamp2d(u, a.gamma, T(1e-8), T(1e-10));
+}
+
+template <typename T, int NX, int MODE, bool COMB>
 __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
@@ -76,17 +114,35 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
   const long long base = (long long)run * a.ss + (long long)y0 * NX;
+  const double d = COMB ? (a.dt_dev != nullptr ? a.dt_dev[run] : a.dt) : 0.0;
 #pragma unroll
   for (int pi = 0; pi < PPT; ++pi) {
     const int p = tid + pi * NT;
     if (p >= 2 * NX) continue;
     const int line = p / NX, x = p - line * NX;
+    const int y = y0 + line;
     T q[4];
+    if constexpr (COMB) {
+      hyb_combine(a, base + p, y, NX, d, true, q);
+    } else {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) q[c] = a.in[c * a.fs + base + p];
+      for (int c = 0; c < 4; ++c) q[c] = a.in[c * a.fs + base + p];
+    }
     if constexpr (MODE == HX_RHS) {
-      const int y = y0 + line;
-      if (a.bc && (y == 0 || y == a.ny - 1)) wall_row(q, a.in + base + p + (y == 0 ? NX : -NX), a.fs);
+      if (a.bc && (y == 0 || y == a.ny - 1)) {
+        const int off = y == 0 ? NX : -NX;
+        if constexpr (COMB) {
+          // the adjacent (interior) row's stage state, recomputed: stages
+          // 1-3 do not read the accumulator, so no other thread's write races
+          T nb[4];
+          hyb_combine(a, base + p + off, y + (y == 0 ? 1 : -1), NX, d, false, nb);
+          q[0] = nb[0];
+          q[2] = T(0);
+          q[3] = nb[3];
+        } else {
+          wall_row(q, a.in + base + p + off, a.fs);
+        }
+      }
       // phrike/equations.py:104-134
       const T rho = q[0];
       const T ux = q[1] / rho, uy = q[2] / rho;
@@ -280,12 +336,24 @@ struct PlanHyb : PlanBase {
     a.bc = bc;
     return a;
   }
-  template <int MODE>
+  template <int MODE, bool COMB = false>
   void hx(int nb, const HxArgs<T>& a, hipStream_t st) const {
     dim3 grid(ny / 2, nb);
-    KScope ks(MODE == HX_RHS ? "h_xderiv" : "h_xfilter", st);
-    PH_SWITCH_N(nx, hipLaunchKernelGGL((k_hx<T, N_, MODE>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
+    KScope ks(MODE == HX_RHS ? (COMB ? "h_xderiv_rk" : "h_xderiv") : (COMB ? "h_xfilter_rk" : "h_xfilter"), st);
+    PH_SWITCH_N(nx, hipLaunchKernelGGL((k_hx<T, N_, MODE, COMB>), grid, dim3(XThreads<T, N_>::value), 0, st, a));
     check_launch("k_hx");
+  }
+  // the combine of stage s (from its derivative r) as the input of hx
+  HxArgs<T> hx_comb_args(int s, const T* r, const T* U, T* acc, double dt, const double* dt_dev, T* dfx,
+                         T* fy, const int* active) const {
+    HxArgs<T> a = hx_args(nullptr, dfx, fy, active);
+    a.rhs = r;
+    a.un = U;
+    a.acc = acc;
+    a.stage = s;
+    a.dt = dt;
+    a.dt_dev = dt_dev;
+    return a;
   }
   void dy_apply(int nb, bool filt_matrix, const T* F, const T* add, T* out, hipStream_t st) const {
     // the derivative's rows sum to zero: split modes may apply it to F - F[row 0]
@@ -306,19 +374,27 @@ struct PlanHyb : PlanBase {
             void* ws, hipStream_t st) const {
     T* r = W(ws, 2);
     T* us = W(ws, 3);
-    for (int s = 1; s <= 4; ++s) {
-      rhs(nb, s == 1 ? U : us, r, active, ws, st);
-      dim3 grid((ncell + NT - 1) / NT, nb);
-      KScope ks("h_rk", st);
-      hipLaunchKernelGGL((k_hrk<T>), grid, dim3(NT), 0, st, (const T*)r, U, out, us, ncell, ncell, 4 * ncell, s,
-                         dt, dt_dev, active, T(gamma), nx, ny, bc);
-      check_launch("k_hrk");
+    T* dfx = W(ws, 0);
+    T* fy = W(ws, 1);
+    // stage 1 from U; stages 2-4 from the combine of the previous stage,
+    // formed inside their x-derivative kernel (accumulator in out)
+    hx<HX_RHS>(nb, hx_args(U, dfx, fy, active), st);
+    dy_apply(nb, false, fy, dfx, r, st);
+    for (int s = 2; s <= 4; ++s) {
+      hx<HX_RHS, true>(nb, hx_comb_args(s - 1, r, U, out, dt, dt_dev, dfx, fy, active), st);
+      dy_apply(nb, false, fy, dfx, r, st);
     }
     if (filt) {
-      // x mask*sigma, then the nodal filter along y
+      // stage-4 combine -> x mask*sigma, then the nodal filter along y
       T* tmp = W(ws, 0);
-      hx<HX_FILTER>(nb, hx_args(out, tmp, nullptr, active), st);
+      hx<HX_FILTER, true>(nb, hx_comb_args(4, r, U, out, dt, dt_dev, tmp, nullptr, active), st);
       dy_apply(nb, true, tmp, nullptr, out, st);
+    } else {
+      dim3 grid((ncell + NT - 1) / NT, nb);
+      KScope ks("h_rk", st);
+      hipLaunchKernelGGL((k_hrk<T>), grid, dim3(NT), 0, st, (const T*)r, U, out, us, ncell, ncell, 4 * ncell, 4,
+                         dt, dt_dev, active, T(gamma), nx, ny, bc);
+      check_launch("k_hrk");
     }
     if (bc) {
       KScope ks("h_wall", st);
