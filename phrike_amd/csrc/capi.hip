@@ -50,17 +50,20 @@ ph::PlanBase* base2(ph_plan2d_t p) {
   return reinterpret_cast<ph::PlanBase*>(p);
 }
 
+// the plan's dynamic type decides (a plan of another kind is an argument error)
 template <typename F>
 void with3(ph_plan3d_t p, F&& f) {
   ph::PlanBase* b = base3(p);
-  if (b->dtype == PH_F64) f(*static_cast<ph::Plan3D<double>*>(b));
-  else f(*static_cast<ph::Plan3D<float>*>(b));
+  if (auto* d = dynamic_cast<ph::Plan3D<double>*>(b)) f(*d);
+  else if (auto* s = dynamic_cast<ph::Plan3D<float>*>(b)) f(*s);
+  else throw ph::Error(PH_ERR_ARG, "not a 3D plan");
 }
 template <typename F>
 void with2(ph_plan2d_t p, F&& f) {
   ph::PlanBase* b = base2(p);
-  if (b->dtype == PH_F64) f(*static_cast<ph::Plan2D<double>*>(b));
-  else f(*static_cast<ph::Plan2D<float>*>(b));
+  if (auto* d = dynamic_cast<ph::Plan2D<double>*>(b)) f(*d);
+  else if (auto* s = dynamic_cast<ph::Plan2D<float>*>(b)) f(*s);
+  else throw ph::Error(PH_ERR_ARG, "not a 2D plan");
 }
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }

@@ -445,8 +445,9 @@ template <typename F>
 void withh(ph_planh_t p, F&& f) {
   if (!p) throw ph::Error(PH_ERR_ARG, "null plan");
   ph::PlanBase* b = reinterpret_cast<ph::PlanBase*>(p);
-  if (b->dtype == PH_F64) f(*static_cast<ph::PlanHyb<double>*>(b));
-  else f(*static_cast<ph::PlanHyb<float>*>(b));
+  if (auto* d = dynamic_cast<ph::PlanHyb<double>*>(b)) f(*d);
+  else if (auto* s = dynamic_cast<ph::PlanHyb<float>*>(b)) f(*s);
+  else throw ph::Error(PH_ERR_ARG, "not a hybrid plan");
 }
 }  // namespace
 

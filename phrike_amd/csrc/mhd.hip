@@ -336,8 +336,9 @@ template <typename F>
 void withm(ph_planmhd3d_t p, F&& f) {
   if (!p) throw ph::Error(PH_ERR_ARG, "null plan");
   auto* b = reinterpret_cast<ph::PlanBase*>(p);
-  if (b->dtype == PH_F64) f(*static_cast<ph::PlanMHD3D<double>*>(b));
-  else f(*static_cast<ph::PlanMHD3D<float>*>(b));
+  if (auto* d = dynamic_cast<ph::PlanMHD3D<double>*>(b)) f(*d);
+  else if (auto* s = dynamic_cast<ph::PlanMHD3D<float>*>(b)) f(*s);
+  else throw ph::Error(PH_ERR_ARG, "not an MHD plan");
 }
 hipStream_t MS(void* s) { return reinterpret_cast<hipStream_t>(s); }
 template <typename P>

@@ -123,3 +123,21 @@ def test_mhd_kernels_vs_reference(emu):
     a = EmuMHD3D(emu, 32, 8, 8, L=(1.0, 1.0, 1.0), filt=(1, 8, 36.0))
     assert field_errors(a.rhs(d["alf_U0"]), d["alf_rhs"]).max() < 1e-12
     assert field_errors(a.step(d["alf_U0"], float(d["alf_dt"])), d["alf_U1"]).max() < 1e-11
+
+
+def test_plan_kind_checked(emu):
+    """A plan handle of another kind is an argument error, not undefined
+    behaviour: every entry point checks the plan's dynamic type."""
+    from emu_lib import Emu3D, EmuMHD3D, P
+    e3 = Emu3D(emu, 8, 8, 8)
+    em = EmuMHD3D(emu, 8, 8, 8)
+    U5 = np.ones((5, 8, 8, 8))
+    U8 = np.ones((8, 8, 8, 8))
+    out8 = np.zeros_like(U8)
+    rc = emu.ph_step_mhd3d(e3.h, P(U8), P(out8), 1e-3, P(em.ws), None)
+    assert rc != 0 and b"not an MHD plan" in emu.ph_last_error()
+    out5 = np.zeros_like(U5)
+    rc = emu.ph_step3d(em.h, P(U5), P(out5), 1e-3, None, P(e3.ws), None)
+    assert rc != 0 and b"not a 3D plan" in emu.ph_last_error()
+    # the right kinds still work
+    assert np.isfinite(e3.step(U5, 1e-3)).all()
