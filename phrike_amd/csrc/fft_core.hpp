@@ -111,11 +111,46 @@ __device__ __forceinline__ void dft8(cx<T>* v) {
   v[7] = e[3] - o3;
 }
 
+// length-16 DFT as 4 x 4: n = j + 4m, k = k1 + 4k2; DFT-4 over m, twiddle
+// exp(DIR*2*pi*i*j*k1/16), DFT-4 over j
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cx<T>* v) {
+  cx<T> t[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    t[j][0] = v[j];
+    t[j][1] = v[j + 4];
+    t[j][2] = v[j + 8];
+    t[j][3] = v[j + 12];
+    dft4<DIR>(t[j]);
+  }
+  const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);  // cos, sin pi/8
+  const T c2 = T(0.70710678118654752440);                                   // cos pi/4
+  // (cos, sin) of 2*pi*q/16 for q = j*k1 in 0..9
+  const T cq[10] = {T(1), c1, c2, s1, T(0), -s1, -c2, -c1, T(-1), -c1};
+  const T sq[10] = {T(0), s1, c2, c1, T(1), c1, c2, s1, T(0), -s1};
+#pragma unroll
+  for (int j = 1; j < 4; ++j)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) {
+      const int q = j * k1;
+      t[j][k1] = cmul(t[j][k1], mk(cq[q], T(DIR) * sq[q]));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cx<T> u[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+    dft4<DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = u[k2];
+  }
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dftR(cx<T>* v) {
   if constexpr (R == 2) dft2<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v);
-  else dft8<DIR>(v);
+  else if constexpr (R == 8) dft8<DIR>(v);
+  else dft16<DIR>(v);
 }
 
 constexpr int ilog2c(int n) { int k = 0; while ((1 << k) < n) ++k; return k; }
@@ -134,12 +169,24 @@ struct Plan {
   }
 };
 
-template <int N, int P>
-struct PassNs {  // product of the radices of passes 0..P-1
-  static constexpr int value = PassNs<N, P - 1>::value * Plan<N>::radix(P - 1);
-};
+// Plan for transforms that run entirely in LDS (the x kernels): as Plan, but
+// N = 1024 takes one radix-16 pass instead of two radix-4 passes, which saves
+// one LDS round trip and barrier per transform (measured 4 % faster 2D steps
+// at 1024^2; at 128 the radix-16 pass leaves too many threads idle).
 template <int N>
-struct PassNs<N, 0> { static constexpr int value = 1; };
+struct PlanX {
+  static constexpr int K = ilog2c(N);
+  static constexpr bool R16 = K % 3 == 1 && K >= 10;
+  static constexpr int npass = R16 ? K / 3 : Plan<N>::npass;
+  static constexpr int radix(int p) { return R16 ? (p == 0 ? 16 : 8) : Plan<N>::radix(p); }
+};
+
+template <int N, int P, class PL = Plan<N>>
+struct PassNs {  // product of the radices of passes 0..P-1
+  static constexpr int value = PassNs<N, P - 1, PL>::value * PL::radix(P - 1);
+};
+template <int N, class PL>
+struct PassNs<N, 0, PL> { static constexpr int value = 1; };
 
 template <int N, int L, int NT>
 struct LastRegs {
@@ -181,10 +228,10 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
 // one Stockham pass, LDS -> LDS (in place; ends with a barrier).  Lines are
 // interleaved with stride LS >= L (padding keeps 16-byte accesses free of
 // bank conflicts when L is not a multiple of 8).
-template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0>
+template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0, class PL = Plan<N>>
 __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  constexpr int R = Plan<N>::radix(P);
-  constexpr int NS = PassNs<N, P>::value;
+  constexpr int R = PL::radix(P);
+  constexpr int NS = PassNs<N, P, PL>::value;
   constexpr int NB = L * (N / R);
   constexpr int PER = (NB + NT - 1) / NT;
   cx<T> v[PER][R];
@@ -208,20 +255,21 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
   lds_barrier();
 }
 
-template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L, int PADB = 0>
+template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L, int PADB = 0,
+          class PL = Plan<N>>
 __device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
   if constexpr (P < PEND) {
-    pass_lds<T, N, L, DIR, NT, P, LS, PADB>(buf, tw, tid);
-    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB>(buf, tw, tid);
+    pass_lds<T, N, L, DIR, NT, P, LS, PADB, PL>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB, PL>(buf, tw, tid);
   }
 }
 
 // Full transform of the L lines in buf (line stride LS), result in buf
 // (natural order).  Caller must have synchronised after filling buf; ends
 // with a barrier.
-template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0>
+template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0, class PL = Plan<N>>
 __device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  passes_lds_range<T, N, L, DIR, NT, 0, Plan<N>::npass, LS, PADB>(buf, tw, tid);
+  passes_lds_range<T, N, L, DIR, NT, 0, PL::npass, LS, PADB, PL>(buf, tw, tid);
 }
 
 // Full transform with the last pass left in registers v[q][r]; output r of

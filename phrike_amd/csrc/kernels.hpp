@@ -81,6 +81,14 @@ struct Keep {
 
 // Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
 // of the block's 2*NX points, which bounds the per-thread flux and FFT state.
+// radix plan of the x kernels' in-LDS transforms: PlanX (a radix-16 pass for
+// log2 N = 3q+1) or the column kernels' Plan
+#ifndef PH_X_RADIX16
+#define PH_X_RADIX16 1
+#endif
+template <int N>
+using XPlan = std::conditional_t<PH_X_RADIX16 != 0, PlanX<N>, Plan<N>>;
+
 #ifndef PH_XT_MAX_F64
 #define PH_XT_MAX_F64 1024
 #endif
@@ -416,7 +424,7 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
       }
     }
     lds_barrier();
-    fft_lds<T, NX, LX, -1, NT, xstride(LX), PH_XPADB>(buf, tw, tid);
+    fft_lds<T, NX, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<NX>>(buf, tw, tid);
     x_split_store<T, NX, DA, LX, xstride(LX), PH_XPADB>(buf, dst + (long long)(g * LX) * a.out_fs, a.out_fs,
                                                         a.mxp, nl, sig, kk, (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();
@@ -842,7 +850,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
   lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
+  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB, XPlan<NX>>(buf, tw, tid);
   const long long base = (long long)slab * a.ss + (long long)y0 * NX;
   const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
 #pragma unroll
@@ -964,7 +972,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   }
   sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
   lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB>(buf, tw, tid);
+  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB, XPlan<NX>>(buf, tw, tid);
   const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
   T q[XJ][2][5];
 #pragma unroll
