@@ -181,7 +181,9 @@ int ph_runs_advance(int nbatch, double* t_dev, const double* t_end_dev, const do
  * Dense LGL collocation derivative on matrix cores:
  *   out[b, i, x] = sum_j D[i, j] F[b, j, x]      (b < nbatch)
  * and, when add != NULL, the hybrid-RHS epilogue out = -(add + D.F).
- * F, add, out are (nbatch, ny, nx); D/Dhi/Dlo are row-major ny x ny.
+ * F, add, out are (nbatch, ny, nx); D/Dhi/Dlo are row-major ny x ny.  out
+ * must not overlap F (every output row reads the whole F line; an overlap is
+ * an argument error); add may be out.
  * mode PH_GLL_F64: FP64 MFMA with D (float64); data float64.
  * mode PH_GLL_SPLIT_F32: FP32 MFMA, D split as Dhi + Dlo (float32 each,
  *   Dhi = fp32(D), Dlo = fp32(D - Dhi)); data float32.

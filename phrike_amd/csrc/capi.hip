@@ -311,7 +311,17 @@ int ph_runs_advance(int nbatch, double* t_dev, const double* t_end_dev, const do
 
 int ph_gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
                  const void* F, const void* add, void* out, void* stream) {
-  return guard([&] { ph::gll_apply(mode, ny, nx, nbatch, D, Dhi, Dlo, F, add, out, S(stream)); });
+  return guard([&] {
+    // every output row reads every row of its F line: no in-place apply
+    if (F && out && ny > 0 && nx > 0 && nbatch > 0) {
+      const size_t es = (mode & ~PH_GLL_OFFSET) == PH_GLL_SPLIT_F32 ? 4 : 8;
+      const size_t n = size_t(nbatch) * size_t(ny) * size_t(nx) * es;
+      const char* f0 = static_cast<const char*>(F);
+      const char* o0 = static_cast<const char*>(out);
+      if (f0 < o0 + n && o0 < f0 + n) throw ph::Error(PH_ERR_ARG, "F and out must not overlap");
+    }
+    ph::gll_apply(mode, ny, nx, nbatch, D, Dhi, Dlo, F, add, out, S(stream));
+  });
 }
 
 }  // extern "C"

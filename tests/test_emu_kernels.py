@@ -141,3 +141,18 @@ def test_plan_kind_checked(emu):
     assert rc != 0 and b"not a 3D plan" in emu.ph_last_error()
     # the right kinds still work
     assert np.isfinite(e3.step(U5, 1e-3)).all()
+
+
+def test_gll_apply_rejects_in_place(emu):
+    """ph_gll_apply with out overlapping F is an argument error (each output
+    row reads the whole line of F)."""
+    import ctypes
+    from emu_lib import P
+    F = np.ones((2, 16, 16))
+    D = np.eye(16)
+    emu.ph_gll_apply.restype = ctypes.c_int
+    rc = emu.ph_gll_apply(0, 16, 16, 2, P(D), None, None, P(F), None, P(F[1:]), None)
+    assert rc != 0 and b"must not overlap" in emu.ph_last_error()
+    out = np.zeros_like(F)
+    assert emu.ph_gll_apply(0, 16, 16, 2, P(D), None, None, P(F), None, P(out), None) == 0
+    np.testing.assert_array_equal(out, F)
