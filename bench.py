@@ -90,7 +90,9 @@ def fused_step_units(n: int, prec: str = "double") -> float:
 
 
 def cpu_baseline(n: int, steps: int):
-    """Time the oracle's step on the host: `steps` steps of TGV n^3 FP64."""
+    """Time the oracle's step on the host: `steps` steps of TGV n^3 FP64, and
+    its compute_dt (vectorised numpy; the reference itself, without numba,
+    takes the CFL maximum in a pure-Python loop over the cells)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import spectral_np as sn
     from phrike_amd.initial_conditions import taylor_green_3d
@@ -100,13 +102,17 @@ def cpu_baseline(n: int, steps: int):
                         filter_params={"enabled": True, "p": 16, "alpha": 36.0}, fft_workers=cores)
     s = sn.Solver3DOracle(g, sn.Euler3DOracle(1.4), cfl=0.4)
     U = taylor_green_3d(n, n, n)
+    t0 = time.perf_counter()
     dt = s.compute_dt(U)
+    t_dt = time.perf_counter() - t0
     g.fftn(U[0])  # plan warm-up
     t0 = time.perf_counter()
     for _ in range(steps):
         U = s.step(U, dt)
     el = time.perf_counter() - t0
     return {"value": n ** 3 * steps / el / 1e6, "unit": "Mcell-updates/s", "cores": cores,
+            "cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "compute_dt_s": round(t_dt, 3),
             "kind": "port",
             "sample": f"oracle (numpy/{sn.FFT_LIBRARY} restatement of the reference NumPy path) "
                       f"TGV {n}^3 FP64, {steps} RK4 step(s) incl. post-step filter, "
