@@ -546,7 +546,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--precision", choices=["double", "single"], default="double")
     ap.add_argument("--cpu-size", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
