@@ -30,11 +30,18 @@ is defined from the reference's working pieces, each restated here:
 Pinning (tests/test_oracle_hybrid.py): dx against closed-form derivatives of
 trigonometric polynomials and against the Fourier Grid2D oracle (pinned by
 the reference's goldens) on rows; D against exact polynomial derivatives; the
-nodal filter against its modal definition; the RK4/clamp/filter composition
-against the Fourier 2D solver oracle on states uniform along y (where D Fy
-and the nodal filter vanish); the wall rule entry by entry.  Beyond these
-per-operator checks the composition is "parity unpinned": the reference has
-no working solver for this grid.
+nodal filter against its modal definition; the wall rule entry by entry; the
+RK4/clamp/filter composition against the Fourier 2D solver oracle on states
+uniform along y.  The D Fy leg on y-varying states: the RHS of an entropy
+wave (rho = 1 + eps P(y) + delta sin(kx) at constant u, v, p) and of
+x-uniform states whose fluxes are polynomials of degree < Ny (with and
+without boundary rows for the wall rule to overwrite) against exact
+derivatives, and one RK4 step of the entropy wave against its exact RK4
+polynomial.  The wall-rule composition (rule applied to every RHS input, to
+the step's base state in place, and after the filter) against the
+reference's own 1D Legendre solver (Grid1D basis="legendre", bc="wall",
+_compute_rhs and _rk4_step; golden tests/golden/wall1d.npz), of which an
+x-uniform hybrid state with zero x momentum is the restriction.
 """
 from __future__ import annotations
 

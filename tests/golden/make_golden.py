@@ -27,6 +27,10 @@ composition the closed-form checks cannot pin are recorded:
       1635-1684): the circularly polarised Alfven wave on 32x8x8 (RHS, 1 and 5
       steps) and a band-limited random state with a non-solenoidal B on 16^3
       (fluxes, RHS, projection, CFL speed, 1 and 3 steps, conserved sums).
+* F10 1D Euler on the LGL grid with the wall rule (Grid1D basis="legendre",
+      bc="wall"; phrike/grid.py:144-738, phrike/solver.py:33-131): one RHS
+      and 1 / 5 RK4 steps (clamp, filter, wall rule) with the filter on and
+      off: the x-uniform restriction of the hybrid Fourier-x / LGL-y path.
 
 The outputs are plain ``.npz`` data (no pickles).  numba, pyfftw and finufft
 are absent here, so the reference runs its plain numpy/scipy branches; the
@@ -292,13 +296,57 @@ def f9(grid_m, eq_m, sol_m, out):
     print("F9", flush=True)
 
 
+def _wall1d_state(x, L):
+    # smooth, y-varying, with non-zero momentum at both walls (so the wall
+    # rule changes the boundary nodes of every RHS input)
+    s = x / L
+    rho = 1.0 + 0.2 * np.sin(2.0 * np.pi * s) + 0.05 * np.cos(5.0 * np.pi * s)
+    u = 0.3 * np.sin(np.pi * s + 0.4) + 0.05
+    p = 1.0 + 0.1 * np.cos(3.0 * np.pi * s)
+    return rho, u, p
+
+
+def f10(grid_m, eq_m, sol_m, out):
+    """1D Euler on the Legendre-Gauss-Lobatto grid with the wall boundary rule
+    (Grid1D basis="legendre", bc="wall"; phrike/grid.py:144-738,
+    phrike/solver.py:33-131): one _compute_rhs and 1 / 5 calls of _rk4_step
+    (RK4, clamp, filter, wall rule) with the filter on and off, at N = 16, 33
+    and 64.  An x-uniform hybrid Fourier-x / LGL-y state with zero x momentum
+    is this 1D problem along y."""
+    res = {}
+    for n in (16, 33, 64):
+        L = 1.0
+        for ftag, fp in (("f", dict(FILT)), ("nof", None)):
+            g = grid_m.Grid1D(n, L, basis="legendre", bc="wall", dealias=False, filter_params=fp)
+            e = eq_m.EulerEquations1D(gamma=1.4)
+            x = np.asarray(g.x)
+            rho, u, p = _wall1d_state(x, L)
+            U0 = e.conservative(rho, u, p)
+            key = f"N{n}_{ftag}"
+            res[f"{key}_x"] = x
+            res[f"{key}_U0"] = U0.copy()
+            res[f"{key}_rhs0"] = sol_m._compute_rhs(g, e, U0.copy())
+            dt = 0.2 * float(g.dx) / float(np.max(np.abs(u) + np.sqrt(1.4 * p / rho)))
+            res[f"{key}_dt"] = np.float64(dt)
+            U = U0.copy()
+            for k in range(5):
+                U = sol_m._rk4_step(g, e, U, dt)
+                if k == 0:
+                    res[f"{key}_U1"] = U.copy()
+            res[f"{key}_U5"] = U
+    np.savez_compressed(os.path.join(out, "wall1d.npz"), **res, **_meta())
+    print("F10", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-long", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     grid_m, eq_m, sol_m, leg_m = _load_reference()
-    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6", "f9"}
+    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6", "f9", "f10"}
+    if "f10" in only:
+        f10(grid_m, eq_m, sol_m, HERE)
     if "f3" in only:
         f3(grid_m, eq_m, sol_m, HERE)
     if "f5" in only:
