@@ -104,6 +104,40 @@ int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const doubl
 int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, int flags,
                  void* ws, void* stream);
 
+/* Device-resident run loop.  Per-run scalars live on the device:
+ *   rs_dev[f * nruns + run], FP64, f = PH_RS_T (time), PH_RS_DT (next dt),
+ *     PH_RS_TEND (t_end), PH_RS_TSTOP (stop time: the caller's next output or
+ *     checkpoint), PH_RS_DTNEXT, PH_RS_DTUSED (dt of the last step taken);
+ *   rf_dev[f * nruns + run], int32, f = PH_RF_ACTIVE, PH_RF_STEPS;
+ * (nruns = 1 for ph_run3d).  Each of the nsteps steps, for every active run:
+ *   dt = min(dt, t_end - t); U' = step(U, dt); dt_next = cfl*h/max_speed(U');
+ *   t += dt; dt_used = dt; dt = dt_next; steps += 1;
+ *   active = t < t_end - 1e-12 && t + 1e-12 < t_stop
+ * (a non-finite dt retires the run with t = NaN).  Step k reads Ua (k even)
+ * or Ub (k odd) and writes the other; steps of an inactive run do nothing, so
+ * a run's state ends in Ua if it took an even number of steps in this call,
+ * else in Ub.  The caller sets dt to the CFL dt of the starting state
+ * (ph_max_wave_speed*), t_stop, and active = 1 before the call; nothing
+ * synchronises the host, which reads t, steps and active back only when it
+ * wants the state (outputs, checkpoints, the end).  `flags` (ph_run3d):
+ * PH_STEP_BANDLIMITED for the first step when Ua is a previous step's
+ * output; later steps always are.  ph_run2d applies the post-step filter
+ * every step (filter interval 1).  Replaces the run loops of
+ * SpectralSolver3D.run (phrike/solver.py:1475-1484) and SpectralSolver2D.run
+ * (phrike/solver.py:764-779). */
+#define PH_RS_T 0
+#define PH_RS_DT 1
+#define PH_RS_TEND 2
+#define PH_RS_TSTOP 3
+#define PH_RS_DTNEXT 4
+#define PH_RS_DTUSED 5
+#define PH_RS_N 6
+#define PH_RF_ACTIVE 0
+#define PH_RF_STEPS 1
+#define PH_RF_N 2
+int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, int* rf_dev, double cfl, int flags,
+             void* ws, void* stream);
+
 /* max over cells of max(|ux|,|uy|,|uz|) + sqrt(gamma p / rho) -> max_dev[0]
  * (FP64), and when dt_dev is not NULL dt_dev[0] = cfl*min(dx,dy,dz)/max (1e-6
  * if max <= 0).  Replaces EulerEquations3D.max_wave_speed
@@ -158,6 +192,11 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
  * 137-147) and SpectralSolver2D.compute_dt (phrike/solver.py:613-620). */
 int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max_dev,
                         double* dt_dev, double cfl, const int* active_dev, void* ws, void* stream);
+
+/* The device-resident run loop (see ph_run3d) for nbatch runs of a batched
+ * 2D plan, each with its own scalars. */
+int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, double* rs_dev, int* rf_dev, double cfl,
+             void* ws, void* stream);
 
 /* Per-run FP64 sums of rho, rho*ux, rho*uy, energy -> out4_dev[run*4 + i].
  * Replaces the record() sums of SpectralSolver2D.run

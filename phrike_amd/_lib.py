@@ -16,6 +16,9 @@ LIB_PATH = os.environ.get("PHRIKE_HIP_LIB") or os.path.join(os.path.dirname(os.p
 PH_F64 = 0
 PH_F32 = 1
 PH_STEP_BANDLIMITED = 1
+# device-resident run loop scalars (ph_run3d / ph_run2d)
+PH_RS_T, PH_RS_DT, PH_RS_TEND, PH_RS_TSTOP, PH_RS_DTNEXT, PH_RS_DTUSED, PH_RS_N = range(7)
+PH_RF_ACTIVE, PH_RF_STEPS, PH_RF_N = range(3)
 
 # symbol -> (restype, argtypes); mirrors include/phrike_hip.h
 _SIGNATURES = {
@@ -31,6 +34,8 @@ _SIGNATURES = {
     "ph_deriv3d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "ph_step3d_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_int, c_void_p, c_void_p]),
+    "ph_run3d": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_void_p,
+                         c_void_p]),
     "ph_max_wave_speed3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "ph_conserved_sums3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_plan2d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
@@ -43,6 +48,8 @@ _SIGNATURES = {
                           c_void_p, c_void_p]),
     "ph_max_wave_speed2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                     c_void_p, c_void_p]),
+    "ph_run2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p,
+                         c_void_p]),
     "ph_conserved_sums2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_runs_clip": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_runs_advance": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -13,8 +13,6 @@ extern template struct Plan3D<double>;
 extern template struct Plan3D<float>;
 extern template struct Plan2D<double>;
 extern template struct Plan2D<float>;
-void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
-               const void* F, const void* add, void* out, hipStream_t st);
 }  // namespace ph
 
 namespace {
@@ -67,6 +65,7 @@ void with2(ph_plan2d_t p, F&& f) {
 }
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
 }  // namespace
 
 namespace ph {
@@ -159,10 +158,11 @@ int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const do
                  void* ws, void* stream) {
   return guard([&] {
     need(U && out && ws, "null buffer");
-    need(U != out, "U and out must not overlap");
     need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      const size_t n = 5ull * size_t(p.ncell) * sizeof(R);
+      need(!ph::ranges_overlap(U, n, out, n), "U and out must not overlap");
       p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, ws, S(stream),
              (flags & PH_STEP_BANDLIMITED) != 0);
     });
@@ -172,6 +172,32 @@ int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const do
 int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, void* ws,
               void* stream) {
   return ph_step3d_ex(plan, U, out, dt, dt_dev, 0, ws, stream);
+}
+
+int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, int* rf_dev, double cfl, int flags,
+             void* ws, void* stream) {
+  return guard([&] {
+    need(Ua && Ub && rs_dev && rf_dev && ws, "null buffer");
+    need(nsteps >= 1, "nsteps must be >= 1");
+    need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      const size_t n = 5ull * size_t(p.ncell) * sizeof(R);
+      need(!ph::ranges_overlap(Ua, n, Ub, n), "Ua and Ub must not overlap");
+      hipStream_t st = S(stream);
+      for (int k = 0; k < nsteps; ++k) {
+        R* src = static_cast<R*>(k % 2 ? Ub : Ua);
+        R* dst = static_cast<R*>(k % 2 ? Ua : Ub);
+        hipLaunchKernelGGL(ph::k_run_clip, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, (const int*)rf_dev);
+        ph::check_launch("k_run_clip");
+        // steps after the first take a previous step's (band-limited) output
+        p.step(src, dst, 0.0, rs_dev + ph::RS_DT, ws, st, k > 0 || (flags & PH_STEP_BANDLIMITED) != 0, rf_dev);
+        p.max_speed(dst, nullptr, rs_dev + ph::RS_DTNEXT, cfl, ws, st, rf_dev);
+        hipLaunchKernelGGL(ph::k_run_advance, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, rf_dev);
+        ph::check_launch("k_run_advance");
+      }
+    });
+  });
 }
 
 int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double* dt_dev,
@@ -255,10 +281,11 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
               const int* active_dev, int apply_filter, void* ws, void* stream) {
   return guard([&] {
     need(U && out && ws, "null buffer");
-    need(U != out, "U and out must not overlap");
     with2(plan, [&](auto& p) {
       PH_R2;
       p.check_nb(nbatch);
+      const size_t n = 4ull * size_t(nbatch) * size_t(p.ncell) * sizeof(R);
+      need(!ph::ranges_overlap(U, n, out, n), "U and out must not overlap");
       p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev,
              apply_filter != 0, ws, S(stream));
     });
@@ -273,6 +300,33 @@ int ph_max_wave_speed2d(ph_plan2d_t plan, int nbatch, const void* U, double* max
       PH_R2;
       p.check_nb(nbatch);
       p.max_speed(nbatch, static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, S(stream), active_dev);
+    });
+  });
+}
+
+int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, double* rs_dev, int* rf_dev, double cfl,
+             void* ws, void* stream) {
+  return guard([&] {
+    need(Ua && Ub && rs_dev && rf_dev && ws, "null buffer");
+    need(nsteps >= 1, "nsteps must be >= 1");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      const size_t n = 4ull * size_t(nbatch) * size_t(p.ncell) * sizeof(R);
+      need(!ph::ranges_overlap(Ua, n, Ub, n), "Ua and Ub must not overlap");
+      hipStream_t st = S(stream);
+      const int nblk = (nbatch + ph::NT - 1) / ph::NT;
+      const int* act = rf_dev + ph::RF_ACTIVE * nbatch;
+      for (int k = 0; k < nsteps; ++k) {
+        const R* src = static_cast<const R*>(k % 2 ? Ub : Ua);
+        R* dst = static_cast<R*>(k % 2 ? Ua : Ub);
+        hipLaunchKernelGGL(ph::k_run_clip, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, (const int*)rf_dev);
+        ph::check_launch("k_run_clip");
+        p.step(nbatch, src, dst, 0.0, rs_dev + ph::RS_DT * nbatch, act, true, ws, st);
+        p.max_speed(nbatch, dst, nullptr, rs_dev + ph::RS_DTNEXT * nbatch, cfl, ws, st, act);
+        hipLaunchKernelGGL(ph::k_run_advance, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, rf_dev);
+        ph::check_launch("k_run_advance");
+      }
     });
   });
 }

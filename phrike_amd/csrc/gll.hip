@@ -48,7 +48,9 @@ template <int EPI>
 __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_f64(const double* __restrict__ D,
                                                            const double* __restrict__ F,
                                                            const double* __restrict__ add,
-                                                           double* __restrict__ out, int ny, int nx) {
+                                                           double* __restrict__ out, int ny, int nx,
+                                                           const int* __restrict__ active, int group) {
+  if (active != nullptr && active[blockIdx.z / group] == 0) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int i0 = (blockIdx.y * GLL_WAVES + wave) * 16;
@@ -87,7 +89,9 @@ __global__ __launch_bounds__(64 * GLL_WAVES) void k_gll_split(const float* __res
                                                              const T* __restrict__ F,
                                                              const T* __restrict__ add,
                                                              T* __restrict__ out, int ny, int nx,
-                                                             int offset) {
+                                                             int offset, const int* __restrict__ active,
+                                                             int group) {
+  if (active != nullptr && active[blockIdx.z / group] == 0) return;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int i0 = (blockIdx.y * GLL_WAVES + wave) * 16;
@@ -147,7 +151,8 @@ constexpr int GKC = 16;  // k per staged chunk
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gll_f64_tiled(const double* __restrict__ D, const double* __restrict__ F,
                                                       const double* __restrict__ add, double* __restrict__ out,
-                                                      int ny, int nx) {
+                                                      int ny, int nx, const int* __restrict__ active, int group) {
+  if (active != nullptr && active[blockIdx.z / group] == 0) return;
   __shared__ double As[GKC][GT];
   __shared__ double Bs[GKC][GT];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -213,7 +218,9 @@ __global__ __launch_bounds__(256) void k_gll_f64_tiled(const double* __restrict_
 template <typename T, int EPI>
 __global__ __launch_bounds__(256) void k_gll_split_tiled(const float* __restrict__ Dhi, const float* __restrict__ Dlo,
                                                         const T* __restrict__ F, const T* __restrict__ add,
-                                                        T* __restrict__ out, int ny, int nx, int offset) {
+                                                        T* __restrict__ out, int ny, int nx, int offset,
+                                                        const int* __restrict__ active, int group) {
+  if (active != nullptr && active[blockIdx.z / group] == 0) return;
   constexpr bool LO = sizeof(T) == 8;  // FP64 data: G = Ghi + Glo
   __shared__ float Ah[GKC][GT], Al[GKC][GT], Bh[GKC][GT];
   __shared__ float Bl[LO ? GKC : 1][LO ? GT : 1];
@@ -308,7 +315,9 @@ __global__ __launch_bounds__(256) void k_gll_split_tiled(const float* __restrict
 namespace ph {
 
 void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
-               const void* F, const void* add, void* out, hipStream_t st) {
+               const void* F, const void* add, void* out, hipStream_t st, const int* active, int group) {
+  if (active != nullptr && (group < 1 || nbatch % group != 0))
+    throw Error(PH_ERR_ARG, "active groups must divide the batch");
   if (ny < 16 || nx < 16 || ny % 16 || nx % 16 || nbatch < 1 || nbatch > 65535)
     throw Error(PH_ERR_ARG, "ny, nx must be positive multiples of 16; nbatch in [1, 65535]");
   if (!F || !out) throw Error(PH_ERR_ARG, "null buffer");
@@ -323,27 +332,27 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
       case PH_GLL_F64:
         if (!D) throw Error(PH_ERR_ARG, "null D");
         if (epi) k_gll_f64_tiled<1><<<grid, block, 0, st>>>((const double*)D, (const double*)F, (const double*)add,
-                                                            (double*)out, ny, nx);
+                                                            (double*)out, ny, nx, active, group);
         else k_gll_f64_tiled<0><<<grid, block, 0, st>>>((const double*)D, (const double*)F, nullptr, (double*)out,
-                                                        ny, nx);
+                                                        ny, nx, active, group);
         break;
       case PH_GLL_SPLIT_F32:
         if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
         if (epi) k_gll_split_tiled<float, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                                      (const float*)F, (const float*)add,
-                                                                     (float*)out, ny, nx, offset);
+                                                                     (float*)out, ny, nx, offset, active, group);
         else k_gll_split_tiled<float, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                                  (const float*)F, nullptr, (float*)out, ny, nx,
-                                                                 offset);
+                                                                 offset, active, group);
         break;
       case PH_GLL_SPLIT_F64IO:
         if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
         if (epi) k_gll_split_tiled<double, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                                       (const double*)F, (const double*)add,
-                                                                      (double*)out, ny, nx, offset);
+                                                                      (double*)out, ny, nx, offset, active, group);
         else k_gll_split_tiled<double, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                                   (const double*)F, nullptr, (double*)out, ny,
-                                                                  nx, offset);
+                                                                  nx, offset, active, group);
         break;
       default:
         throw Error(PH_ERR_ARG, "unknown mode");
@@ -358,32 +367,32 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
       if (!D) throw Error(PH_ERR_ARG, "null D");
       if (epi)
         k_gll_f64<1><<<grid, block, 0, st>>>((const double*)D, (const double*)F, (const double*)add,
-                                             (double*)out, ny, nx);
+                                             (double*)out, ny, nx, active, group);
       else
         k_gll_f64<0><<<grid, block, 0, st>>>((const double*)D, (const double*)F, nullptr, (double*)out,
-                                             ny, nx);
+                                             ny, nx, active, group);
       break;
     case PH_GLL_SPLIT_F32:
       if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
       if (epi)
         k_gll_split<float, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                       (const float*)F, (const float*)add, (float*)out,
-                                                      ny, nx, offset);
+                                                      ny, nx, offset, active, group);
       else
         k_gll_split<float, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                       (const float*)F, nullptr, (float*)out, ny, nx,
-                                                      offset);
+                                                      offset, active, group);
       break;
     case PH_GLL_SPLIT_F64IO:
       if (!Dhi || !Dlo) throw Error(PH_ERR_ARG, "null Dhi/Dlo");
       if (epi)
         k_gll_split<double, 1><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                        (const double*)F, (const double*)add,
-                                                       (double*)out, ny, nx, offset);
+                                                       (double*)out, ny, nx, offset, active, group);
       else
         k_gll_split<double, 0><<<grid, block, 0, st>>>((const float*)Dhi, (const float*)Dlo,
                                                        (const double*)F, nullptr, (double*)out, ny, nx,
-                                                       offset);
+                                                       offset, active, group);
       break;
     default:
       throw Error(PH_ERR_ARG, "unknown mode");

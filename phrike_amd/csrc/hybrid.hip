@@ -37,8 +37,6 @@
 
 namespace ph {
 
-void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
-               const void* F, const void* add, void* out, hipStream_t st);
 
 enum HxMode { HX_RHS = 0, HX_FILTER = 1 };
 
@@ -355,20 +353,22 @@ struct PlanHyb : PlanBase {
     a.dt_dev = dt_dev;
     return a;
   }
-  void dy_apply(int nb, bool filt_matrix, const T* F, const T* add, T* out, hipStream_t st) const {
+  // runs with active[run] == 0 are skipped: their out is left untouched
+  void dy_apply(int nb, bool filt_matrix, const T* F, const T* add, T* out, const int* active,
+                hipStream_t st) const {
     // the derivative's rows sum to zero: split modes may apply it to F - F[row 0]
     const int mode = (gll_mode != PH_GLL_F64 && !filt_matrix) ? (gll_mode | PH_GLL_OFFSET) : gll_mode;
     KScope ks(filt_matrix ? "h_gll_filter" : "h_gll_d", st);
     gll_apply(mode, ny, nx, 4 * nb, filt_matrix ? (const void*)Fn : (const void*)D,
               filt_matrix ? (const void*)Fnhi : (const void*)Dhi, filt_matrix ? (const void*)Fnlo : (const void*)Dlo,
-              F, add, out, st);
+              F, add, out, st, active, 4);
   }
   // rhs = -(dx Fx + D Fy)
   void rhs(int nb, const T* U, T* out, const int* active, void* ws, hipStream_t st) const {
     T* dfx = W(ws, 0);
     T* fy = W(ws, 1);
     hx<HX_RHS>(nb, hx_args(U, dfx, fy, active), st);
-    dy_apply(nb, false, fy, dfx, out, st);
+    dy_apply(nb, false, fy, dfx, out, active, st);
   }
   void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active, bool filt,
             void* ws, hipStream_t st) const {
@@ -379,16 +379,16 @@ struct PlanHyb : PlanBase {
     // stage 1 from U; stages 2-4 from the combine of the previous stage,
     // formed inside their x-derivative kernel (accumulator in out)
     hx<HX_RHS>(nb, hx_args(U, dfx, fy, active), st);
-    dy_apply(nb, false, fy, dfx, r, st);
+    dy_apply(nb, false, fy, dfx, r, active, st);
     for (int s = 2; s <= 4; ++s) {
       hx<HX_RHS, true>(nb, hx_comb_args(s - 1, r, U, out, dt, dt_dev, dfx, fy, active), st);
-      dy_apply(nb, false, fy, dfx, r, st);
+      dy_apply(nb, false, fy, dfx, r, active, st);
     }
     if (filt) {
       // stage-4 combine -> x mask*sigma, then the nodal filter along y
       T* tmp = W(ws, 0);
       hx<HX_FILTER, true>(nb, hx_comb_args(4, r, U, out, dt, dt_dev, tmp, nullptr, active), st);
-      dy_apply(nb, true, tmp, nullptr, out, st);
+      dy_apply(nb, true, tmp, nullptr, out, active, st);
     } else {
       dim3 grid((ncell + NT - 1) / NT, nb);
       KScope ks("h_rk", st);
@@ -410,7 +410,7 @@ struct PlanHyb : PlanBase {
                        T(gamma), part, (const int*)nullptr);
     check_launch("k_maxspeed_partial(hyb)");
     hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, mx, dt_out,
-                       cfl * std::min(dx, dy_min));
+                       cfl * std::min(dx, dy_min), (const int*)nullptr);
     check_launch("k_max_final(hyb)");
   }
 };
@@ -477,7 +477,10 @@ int ph_planh_destroy(ph_planh_t plan) {
 }
 
 int ph_planh_workspace_bytes(ph_planh_t plan, size_t* bytes) {
-  return hguard([&] { withh(plan, [&](auto& p) { *bytes = p.wsBytes; }); });
+  return hguard([&] {
+    if (!bytes) throw ph::Error(PH_ERR_ARG, "null output");
+    withh(plan, [&](auto& p) { *bytes = p.wsBytes; });
+  });
 }
 
 #define PH_RH using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::PlanHyb<double>>, double, float>
@@ -498,8 +501,9 @@ int ph_step2dh(ph_planh_t plan, int nbatch, const void* U, void* out, double dt,
   return hguard([&] {
     withh(plan, [&](auto& p) {
       PH_RH;
-      if (nbatch < 1 || nbatch > p.batch || !U || !out || !ws || U == out)
-        throw ph::Error(PH_ERR_ARG, "bad arguments");
+      if (nbatch < 1 || nbatch > p.batch || !U || !out || !ws) throw ph::Error(PH_ERR_ARG, "bad arguments");
+      const size_t n = 4ull * size_t(nbatch) * size_t(p.ncell) * sizeof(R);
+      if (ph::ranges_overlap(U, n, out, n)) throw ph::Error(PH_ERR_ARG, "U and out must not overlap");
       p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev, apply_filter != 0,
              ws, reinterpret_cast<hipStream_t>(stream));
     });

@@ -309,7 +309,13 @@ struct XFwdArgs {
   const T* kk;   // kx(m), zero at the Nyquist mode
   T gamma;
   const int* active;  // per-slab flag (batched 2D) or null
+  int act_all;        // 1: active[0] gates every slab (a device-resident 3D run loop)
 };
+
+// is this block's slab switched off?  active[slab], or active[0] for all
+__device__ __forceinline__ bool gated_off(const int* active, int act_all, int slab) {
+  return active != nullptr && active[act_all ? 0 : slab] == 0;
+}
 
 template <int MODE> struct XFwdShape;
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
@@ -450,7 +456,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
-  if (a.active != nullptr && a.active[slab] == 0) return;
+  if (gated_off(a.active, a.act_all, slab)) return;
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
   x_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
@@ -498,6 +504,7 @@ struct ColArgs {
   const T* sig;  // by compact index
   const T* kk;   // by compact index
   const int* active;  // per blockIdx.y flag (batched 2D runs) or null
+  int act_all;        // 1: active[0] gates every block
   // CM_FWDINV_CORR: sigma - 1 of this axis (by compact index), of the column
   // axis (by column) and of the axis of blockIdx.y (by blockIdx.y)
   const T* sm1;
@@ -566,7 +573,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   constexpr int PER = LR::PER, R = LR::R;
   __shared__ cx<T> tw[N];
   __shared__ cx<T> buf[N * TM];
-  if (a.active != nullptr && a.active[blockIdx.y] == 0) return;
+  if (gated_off(a.active, a.act_all, blockIdx.y)) return;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * TM;
   for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
@@ -739,6 +746,7 @@ struct XInvArgs {
   int dt_per_slab;         // 1: one dt per slab (batched 2D runs)
   int keep_state;          // k_xmid stage 4: also write the new state to out
   const int* active;       // per-slab flag (batched 2D) or null
+  int act_all;             // 1: active[0] gates every slab
   T gamma, rho_min, p_min;
   const cx<T>* tw;
 };
@@ -840,7 +848,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   __shared__ cx<T> buf[xbuf_size<NX, NC>()];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
-  if (a.active != nullptr && a.active[slab] == 0) return;
+  if (gated_off(a.active, a.act_all, slab)) return;
   const int y0 = 2 * blockIdx.x;
   PH_XTW_FILL(a.tw, NX, NT, tid);
   const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
@@ -939,7 +947,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
-  if (a.active != nullptr && a.active[slab] == 0) return;
+  if (gated_off(a.active, a.act_all, slab)) return;
   const int y0 = 2 * blockIdx.x;
   XSpec<T, NX, DA, NC> sp;
   sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, NC, tid);
@@ -1083,7 +1091,9 @@ __global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long f
 // one block per slab: reduce nb partials -> out[slab]; optionally dt = cfl*h/max
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_max_final(const double* part, int nb, double* out,
-                                                  double* dt_out, double cfl_h) {
+                                                  double* dt_out, double cfl_h, const int* active) {
+  // an inactive run keeps its previous max and dt
+  if (active != nullptr && active[blockIdx.x] == 0) return;
   __shared__ double red[NT];
   const double* ps = part + (long long)blockIdx.x * nb;
   double m = 0.0;
@@ -1095,8 +1105,11 @@ __global__ __launch_bounds__(NT) void k_max_final(const double* part, int nb, do
     lds_barrier();
   }
   if (threadIdx.x == 0) {
-    if (out) out[blockIdx.x] = red[0];
-    if (dt_out) dt_out[blockIdx.x] = red[0] > 0.0 ? cfl_h / red[0] : 1e-6;
+    const double m = red[0];
+    if (out) out[blockIdx.x] = m;
+    // the reference's compute_dt: 1e-6 when the speed is <= 0, else cfl*h/max,
+    // which is NaN for a NaN speed (a diverged run must not keep stepping)
+    if (dt_out) dt_out[blockIdx.x] = m != m ? m : (m > 0.0 ? cfl_h / m : 1e-6);
   }
 }
 
@@ -1184,14 +1197,52 @@ static __global__ __launch_bounds__(NT) void k_runs_clip(int nb, const double* t
   }
 }
 
+// A run whose dt is not finite (a NaN max speed) retires with t = NaN: the
+// caller reports it as failed (the reference's while t < t_end loop ends the
+// same way, phrike/solver.py:1475).
 static __global__ __launch_bounds__(NT) void k_runs_advance(int nb, double* t, const double* t_end,
                                                      const double* dt, int* active, int* steps) {
   const int r = blockIdx.x * NT + threadIdx.x;
   if (r < nb && active[r]) {
-    const double tn = t[r] + dt[r];
+    const double d = dt[r];
+    const double tn = (d - d == 0.0) ? t[r] + d : __builtin_nan("");
     t[r] = tn;
     steps[r] += 1;
     active[r] = tn < t_end[r] - 1e-12 ? 1 : 0;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Device-resident run loop (phrike/solver.py:1475-1484 for 3D, 764-779 for
+// 2D): per-run scalars in rs[field * nb + run] (RS_* fields, FP64) and
+// rf[field * nb + run] (RF_* fields, int32).  Per step: clip dt to t_end,
+// step with dt, CFL dt of the new state into dt_next, then advance: t += dt,
+// dt_used = dt, dt = dt_next, steps += 1, and the run stays active while
+// t < t_end - 1e-12 and t + 1e-12 < t_stop (t_stop: the caller's next output
+// or checkpoint time, where it wants the state back).  A non-finite dt
+// retires the run with t = NaN.
+// ----------------------------------------------------------------------------
+enum RunScalar { RS_T = 0, RS_DT = 1, RS_TEND = 2, RS_TSTOP = 3, RS_DTNEXT = 4, RS_DTUSED = 5, RS_N = 6 };
+enum RunFlag { RF_ACTIVE = 0, RF_STEPS = 1, RF_N = 2 };
+
+static __global__ __launch_bounds__(NT) void k_run_clip(int nb, double* rs, const int* rf) {
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r < nb && rf[RF_ACTIVE * nb + r]) {
+    const double rem = rs[RS_TEND * nb + r] - rs[RS_T * nb + r];
+    if (rem < rs[RS_DT * nb + r]) rs[RS_DT * nb + r] = rem;
+  }
+}
+
+static __global__ __launch_bounds__(NT) void k_run_advance(int nb, double* rs, int* rf) {
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r < nb && rf[RF_ACTIVE * nb + r]) {
+    const double d = rs[RS_DT * nb + r];
+    const double tn = (d - d == 0.0) ? rs[RS_T * nb + r] + d : __builtin_nan("");
+    rs[RS_T * nb + r] = tn;
+    rs[RS_DTUSED * nb + r] = d;
+    rs[RS_DT * nb + r] = rs[RS_DTNEXT * nb + r];
+    rf[RF_STEPS * nb + r] += 1;
+    rf[RF_ACTIVE * nb + r] = (tn < rs[RS_TEND * nb + r] - 1e-12 && tn + 1e-12 < rs[RS_TSTOP * nb + r]) ? 1 : 0;
   }
 }
 

@@ -296,7 +296,8 @@ struct PlanMHD3D : PlanBase {
     hipLaunchKernelGGL((k_mhd_maxspeed<T>), dim3(nb), dim3(NT), 0, st, U, ncell, T(gamma), Part(ws));
     check_launch("k_mhd_maxspeed");
     const double h = std::min({main.lx / nx, main.ly / ny, main.lz / nz});
-    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, Part(ws), nb, out, dt_out, cfl * h);
+    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, Part(ws), nb, out, dt_out, cfl * h,
+                       (const int*)nullptr);
     check_launch("k_max_final");
   }
   void sums(const T* U, double* out, void* ws, hipStream_t st) const {
@@ -399,9 +400,10 @@ int ph_project_mhd3d(ph_planmhd3d_t plan, void* U, void* ws, void* stream) {
 int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void* ws, void* stream) {
   return mguard([&] {
     if (!U || !out || !ws) throw ph::Error(PH_ERR_ARG, "null buffer");
-    if (U == out) throw ph::Error(PH_ERR_ARG, "U and out must not overlap");
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;
+      const size_t n = 8ull * size_t(p.ncell) * sizeof(R);
+      if (ph::ranges_overlap(U, n, out, n)) throw ph::Error(PH_ERR_ARG, "U and out must not overlap");
       p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, ws, MS(stream));
     });
   });

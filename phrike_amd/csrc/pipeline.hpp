@@ -29,6 +29,13 @@ inline void hip_check(hipError_t e, const char* what) {
 
 inline bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
 
+// do the byte ranges [a, a + na) and [b, b + nb) overlap?
+inline bool ranges_overlap(const void* a, size_t na, const void* b, size_t nb) {
+  const char* pa = static_cast<const char*>(a);
+  const char* pb = static_cast<const char*>(b);
+  return pa < pb + nb && pb < pa + na;
+}
+
 // Retained modes of one axis and its FP64 tables (phrike/grid.py:87-141,
 // 1381-1392): k = 2*pi*(m*(1/(N*d))) with d = L/N in fftfreq order,
 // 2/3 rule |m| <= N//3, sigma = exp(-alpha*(|m|/max(N/2,1))^p).
@@ -190,6 +197,13 @@ void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
 
+// D.U on matrix cores (gll.hip): out[b] = D F[b] for b < nbatch, or the
+// hybrid-RHS epilogue out = -(add + D F) when add is not null.  With `active`,
+// batch element b is skipped (out untouched) when active[b / group] == 0.
+void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
+               const void* F, const void* add, void* out, hipStream_t st, const int* active = nullptr,
+               int group = 1);
+
 // Optional per-kernel timing: when enabled, every pipeline launch is
 // bracketed by two HIP events on its own stream; collect() waits for them and
 // accumulates elapsed milliseconds per kernel tag.
@@ -340,8 +354,10 @@ struct Plan3D : PlanBase {
   long long b_zs() const { return PH_B_MYZ ? (long long)mxp : (long long)ay.nr * mxp; }
   long long b_ys() const { return PH_B_MYZ ? (long long)nz * mxp : (long long)mxp; }
 
-  XFwdArgs<T> xfwd_args(const T* in, int nf, unsigned ikmask, void* ws) const {
+  XFwdArgs<T> xfwd_args(const T* in, int nf, unsigned ikmask, void* ws, const int* gate = nullptr) const {
     XFwdArgs<T> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = in;
     a.in_fs = ncell;
     a.in_ss = (long long)ny * nx;
@@ -359,8 +375,11 @@ struct Plan3D : PlanBase {
     return a;
   }
   // forward x pass
-  void xfwd(int mode, const T* in, int nf, unsigned ikmask, void* ws, hipStream_t st) const {
+  void xfwd(int mode, const T* in, int nf, unsigned ikmask, void* ws, hipStream_t st,
+            const int* gate = nullptr) const {
     XFwdArgs<T> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = in;
     a.in_fs = ncell;
     a.in_ss = (long long)ny * nx;
@@ -382,8 +401,11 @@ struct Plan3D : PlanBase {
     check_launch("k_xfwd");
   }
   // y forward: A -> B
-  void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st, bool unit = false) const {
+  void yfwd(bool div, int nc, int ikflag, void* ws, hipStream_t st, bool unit = false,
+            const int* gate = nullptr) const {
     ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = reinterpret_cast<cx<CE>*>(A(ws));
     a.in_fs = (a_fs()) / PW;
     a.in_bs = ((long long)ny * mxp) / PW;
@@ -405,8 +427,10 @@ struct Plan3D : PlanBase {
     check_launch("k_col(y fwd)");
   }
   // z forward + combine + inverse, in place in B
-  void zpass(bool div, int nc, int ikflag, void* ws, hipStream_t st) const {
+  void zpass(bool div, int nc, int ikflag, void* ws, hipStream_t st, const int* gate = nullptr) const {
     ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
     a.in_bs = b_ys() / PW;
@@ -453,8 +477,10 @@ struct Plan3D : PlanBase {
   }
   // filter correction along z: kept modes times sx sy sz - 1 (after unit x
   // and y passes), in place in B
-  void zcorr(int nc, void* ws, hipStream_t st) const {
+  void zcorr(int nc, void* ws, hipStream_t st, const int* gate = nullptr) const {
     ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
     a.in_bs = b_ys() / PW;
@@ -477,8 +503,10 @@ struct Plan3D : PlanBase {
     check_launch("k_col(z corr)");
   }
   // y inverse: B (compact) -> A (full)
-  void yinv(int nc, void* ws, hipStream_t st) const {
+  void yinv(int nc, void* ws, hipStream_t st, const int* gate = nullptr) const {
     ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = reinterpret_cast<cx<CE>*>(B(ws));
     a.in_fs = (b_fs()) / PW;
     a.in_bs = b_zs() / PW;
@@ -498,8 +526,10 @@ struct Plan3D : PlanBase {
     launch_col<CE, CM_INV>(ny, nz, a, st);
     check_launch("k_col(y inv)");
   }
-  XInvArgs<T> xinv_args(void* ws) const {
+  XInvArgs<T> xinv_args(void* ws, const int* gate = nullptr) const {
     XInvArgs<T> a{};
+    a.active = gate;
+    a.act_all = 1;
     a.in = A(ws);
     a.in_fs = a_fs();
     a.in_ss = (long long)ny * mxp;
@@ -553,16 +583,19 @@ struct Plan3D : PlanBase {
   // FP32 transforms carry only the small correction instead of the whole
   // state, whose round-trip rounding would otherwise accumulate step after
   // step.  FP64 plans always use the direct form (the reference's order).
+  // `gate` (device int, or null): when gate[0] == 0 every kernel of the step
+  // returns at once and out is left as it is (the device-resident run loop
+  // keeps launching steps after its run has stopped)
   void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st,
-            bool bandlimited = false) const {
+            bool bandlimited = false, const int* gate = nullptr) const {
     const bool corr = bandlimited && sizeof(T) == 4;
-    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st);
+    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st, gate);
     for (int s = 1; s <= 4; ++s) {
-      yfwd(true, 5, 0, ws, st);
-      zpass(true, 5, 0, ws, st);
-      yinv(5, ws, st);
+      yfwd(true, 5, 0, ws, st, false, gate);
+      zpass(true, 5, 0, ws, st, gate);
+      yinv(5, ws, st, gate);
       XMidArgs<T> m{};
-      m.inv = xinv_args(ws);
+      m.inv = xinv_args(ws, gate);
       m.inv.nf = 5;
       m.inv.out = out;
       m.inv.un = U;
@@ -570,41 +603,43 @@ struct Plan3D : PlanBase {
       m.inv.dt = dt;
       m.inv.dt_dev = dt_dev;
       m.inv.keep_state = corr ? 1 : 0;
-      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws);
-      m.fwd_last = xfwd_args(nullptr, 5, 0u, ws);
+      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws, gate);
+      m.fwd_last = xfwd_args(nullptr, 5, 0u, ws, gate);
       if (corr) m.fwd_last.sig = dx.one;
       KScope ks(s == 4 ? (corr ? "xmid_last_keep" : "xmid_last") : (s == 1 ? "xmid1" : "xmid23"), st);
       launch_xmid<T, 5, XF_FLUX3>(nx, ny, nz, m, st);
       check_launch("k_xmid");
     }
     // remaining passes of the post-step filter on the new state
-    XInvArgs<T> a = xinv_args(ws);
+    XInvArgs<T> a = xinv_args(ws, gate);
     a.nf = 5;
     a.out = out;
     if (corr) {
-      yfwd(false, 5, 0, ws, st, true);
-      zcorr(5, ws, st);
-      yinv(5, ws, st);
+      yfwd(false, 5, 0, ws, st, true, gate);
+      zcorr(5, ws, st, gate);
+      yinv(5, ws, st, gate);
       KScope ks("xinv_add", st);
       launch_xinv<T, 5, XI_ADD>(nx, ny, nz, a, st);
     } else {
-      yfwd(false, 5, 0, ws, st);
-      zpass(false, 5, 0, ws, st);
-      yinv(5, ws, st);
+      yfwd(false, 5, 0, ws, st, false, gate);
+      zpass(false, 5, 0, ws, st, gate);
+      yinv(5, ws, st, gate);
       KScope ks("xinv_out", st);
       launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
     }
     check_launch("k_xinv");
   }
 
-  void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st) const {
+  void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st,
+                 const int* gate = nullptr) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
     hipLaunchKernelGGL((k_maxspeed_partial<T, 3>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma), part,
-                       (const int*)nullptr);
+                       gate);
     check_launch("k_maxspeed_partial");
     const double h = std::min(lx / nx, std::min(ly / ny, lz / nz));
-    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out, dt_out, cfl * h);
+    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out, dt_out, cfl * h,
+                       gate);
     check_launch("k_max_final");
   }
   void sums(const T* U, double* out, void* ws, hipStream_t st) const {
@@ -807,7 +842,8 @@ struct Plan2D : PlanBase {
                        active);
     check_launch("k_maxspeed_partial(2d)");
     const double h = std::min(lx / nx, ly / ny);
-    hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out, dt_out, cfl * h);
+    hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out, dt_out, cfl * h,
+                       active);
     check_launch("k_max_final(2d)");
   }
   void sums(int nb, const T* U, double* out, void* ws, hipStream_t st) const {

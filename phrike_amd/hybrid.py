@@ -183,6 +183,17 @@ class HybridSolver2D:
         return self.cfl * min(self.grid.dx, self.grid.dy_min) / s
 
     def step_into(self, Ub, out, dt: float, dt_dev=None, active_dev=None, apply_filter: bool = True) -> None:
+        """out = one step of every run of Ub (B, 4, Ny, Nx), caller-owned; runs
+        with active_dev == 0 are left untouched in out."""
+        from .solver import _check_pair, _check_scalars
+        g = self.grid
+        if not isinstance(Ub, torch.Tensor) or Ub.dim() != 4 or tuple(Ub.shape[1:]) != (4, g.Ny, g.Nx) \
+                or Ub.dtype != g.dtype or Ub.device.type != "cuda" or not Ub.is_contiguous() \
+                or not 1 <= Ub.shape[0] <= g.batch:
+            raise ValueError(f"Ub must be a contiguous (B <= {g.batch}, 4, {g.Ny}, {g.Nx}) {g.dtype} GPU tensor")
+        _check_pair(Ub, out, "out")
+        _check_scalars(dt_dev, Ub.shape[0], torch.float64, Ub.device, "dt_dev")
+        _check_scalars(active_dev, Ub.shape[0], torch.int32, Ub.device, "active_dev")
         plan, ws, st = self._ctx()
         _lib.call("ph_step2dh", plan, Ub.shape[0], _lib.ptr(Ub), _lib.ptr(out), float(dt), _lib.ptr(dt_dev),
                   _lib.ptr(active_dev), int(apply_filter), ws, st)

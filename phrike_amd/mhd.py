@@ -167,6 +167,11 @@ class SpectralSolverMHD3D:
     _fixed_step = step
 
     def step_into(self, U, out, dt: float) -> None:
+        """out = one step of U with caller-owned output (checked: same shape,
+        dtype and device as U, contiguous, no shared memory)."""
+        from .solver import _check_pair
+        U = self._state(U)
+        _check_pair(U, out, "out")
         _lib.call("ph_step_mhd3d", self._plan, _lib.ptr(U), _lib.ptr(out), float(dt), _lib.ptr(self._ws),
                   self._st())
 
@@ -177,7 +182,8 @@ class SpectralSolverMHD3D:
         """phrike/solver.py:1095-1163: the initial state is projected, dt from
         the CFL condition after every step, the last step clipped to t_end,
         conserved quantities and max |div B| recorded at t0, every
-        output_interval and at the end."""
+        output_interval and at the end.  Callbacks get a copy of the state
+        (the loop reuses two buffers)."""
         if tracers is not None:
             raise NotImplementedError("tracers are not part of the hip backend")
         self.U = self._project_B(U0)
@@ -197,7 +203,7 @@ class SpectralSolverMHD3D:
 
         record()
         if on_output is not None:
-            on_output(self.t, self.U)
+            on_output(self.t, self.U.clone())
         dt = self.compute_dt(self.U)
         step_count = 0
         buf = torch.empty_like(self.U)
@@ -207,13 +213,13 @@ class SpectralSolverMHD3D:
             self.U, buf = buf, self.U
             self.t += dt
             if on_step is not None:
-                on_step(step_count, dt, self.U)
+                on_step(step_count, dt, self.U.clone())
             dt = self.compute_dt(self.U)
             step_count += 1
             if self.t + 1e-12 >= next_output:
                 record()
                 if on_output is not None:
-                    on_output(self.t, self.U)
+                    on_output(self.t, self.U.clone())
                 next_output += output_interval
             if outdir and self.t + 1e-12 >= next_ckpt:
                 from .io import save_solution_snapshot
@@ -221,5 +227,5 @@ class SpectralSolverMHD3D:
                 next_ckpt += checkpoint_interval
         record()
         if on_output is not None:
-            on_output(self.t, self.U)
+            on_output(self.t, self.U.clone())
         return history

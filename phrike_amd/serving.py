@@ -64,6 +64,13 @@ class RunResult:
     def latency(self) -> float:
         return self.finished - self.arrival
 
+    @property
+    def failed(self) -> bool:
+        """The run diverged: a NaN wave speed gave a NaN dt, which retires the
+        run with t = NaN (the reference's while t < t_end loop ends the same
+        way, phrike/solver.py:764)."""
+        return not np.isfinite(self.t_final)
+
 
 def slot_bytes(n: int, precision: str = "single") -> int:
     """HBM per slot: two state buffers + the per-run share of the plan workspace

@@ -79,11 +79,12 @@ class Emu2D:
         if rc != 0:
             raise RuntimeError(self.lib.ph_last_error().decode())
 
-    def step(self, U, dt, filt=True):
+    def step(self, U, dt, filt=True, active=None, out=None, dt_dev=None):
         U = np.ascontiguousarray(U, dtype=self.dtype)
-        out = np.zeros_like(U)
+        out = np.zeros_like(U) if out is None else out
         nb = U.shape[0] if U.ndim == 4 else 1
-        self._chk(self.lib.ph_step2d(self.h, nb, P(U), P(out), dt, None, None, int(filt), P(self.ws), None))
+        self._chk(self.lib.ph_step2d(self.h, nb, P(U), P(out), dt, P(dt_dev), P(active), int(filt),
+                                     P(self.ws), None))
         return out
 
     def rhs(self, U):
@@ -92,6 +93,13 @@ class Emu2D:
         nb = U.shape[0] if U.ndim == 4 else 1
         self._chk(self.lib.ph_rhs2d(self.h, nb, P(U), P(out), P(self.ws), None))
         return out
+
+    def dt(self, U, cfl, active=None):
+        U = np.ascontiguousarray(U, dtype=self.dtype)
+        nb = U.shape[0] if U.ndim == 4 else 1
+        m, d = np.zeros(nb), np.zeros(nb)
+        self._chk(self.lib.ph_max_wave_speed2d(self.h, nb, P(U), P(m), P(d), cfl, P(active), P(self.ws), None))
+        return m, d
 
 
 class EmuHybrid:
@@ -120,11 +128,11 @@ class EmuHybrid:
         self._chk(self.lib.ph_rhs2dh(self.h, nb, P(U), P(out), P(self.ws), None))
         return out
 
-    def step(self, U, dt, filt=True):
+    def step(self, U, dt, filt=True, active=None, out=None):
         U = np.ascontiguousarray(U, dtype=self.dtype)
-        out = np.zeros_like(U)
+        out = np.zeros_like(U) if out is None else out
         nb = U.shape[0] if U.ndim == 4 else 1
-        self._chk(self.lib.ph_step2dh(self.h, nb, P(U), P(out), dt, None, None, int(filt), P(self.ws), None))
+        self._chk(self.lib.ph_step2dh(self.h, nb, P(U), P(out), dt, None, P(active), int(filt), P(self.ws), None))
         return out
 
     def dt(self, U, cfl):

@@ -8,11 +8,12 @@
 
 namespace ph {
 void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* Dhi, const void* Dlo,
-               const void* F, const void* add, void* out, hipStream_t) {
+               const void* F, const void* add, void* out, hipStream_t, const int* active, int group) {
   const bool offset = (mode & PH_GLL_OFFSET) != 0;
   mode &= ~PH_GLL_OFFSET;
   const long long plane = (long long)ny * nx;
-  for (int b = 0; b < nbatch; ++b)
+  for (int b = 0; b < nbatch; ++b) {
+    if (active != nullptr && active[b / group] == 0) continue;
     for (int i = 0; i < ny; ++i)
       for (int x = 0; x < nx; ++x) {
         const long long o = b * plane + (long long)i * nx + x;
@@ -51,5 +52,6 @@ void gll_apply(int mode, int ny, int nx, int nbatch, const void* D, const void* 
           static_cast<double*>(out)[o] = add ? -(static_cast<const double*>(add)[o] + s) : s;
         }
       }
+  }
 }
 }  // namespace ph

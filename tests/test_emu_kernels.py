@@ -156,3 +156,86 @@ def test_gll_apply_rejects_in_place(emu):
     out = np.zeros_like(F)
     assert emu.ph_gll_apply(0, 16, 16, 2, P(D), None, None, P(F), None, P(out), None) == 0
     np.testing.assert_array_equal(out, F)
+
+
+def test_hybrid_inactive_run_untouched_with_filter(emu):
+    """A masked run (active 0) of a batched hybrid step keeps its output
+    buffer bit for bit with the filter on (the nodal-filter apply skips it),
+    and the active runs equal their steps alone."""
+    from emu_lib import EmuHybrid
+    from oracle.hybrid_np import HybridGridOracle
+    from phrike_amd.initial_conditions import khi_2d_gll
+    FP = {"enabled": True, "p": 16, "alpha": 36.0}
+    og = HybridGridOracle(16, 16, filter_params=FP)
+    e = EmuHybrid(emu, 16, 16, og.D, og.Fn, og.dy_min, batch=3)
+    U = np.stack([khi_2d_gll(16, og.y, seed=s) for s in (1, 2, 3)])
+    active = np.array([1, 0, 1], dtype=np.int32)
+    sentinel = np.full_like(U, 7.25)
+    out = e.step(U, 1e-3, active=active, out=sentinel.copy())
+    assert np.array_equal(out[1], sentinel[1])
+    for b in (0, 2):
+        assert np.array_equal(out[b], e.step(U[b:b + 1], 1e-3)[0])
+
+
+def test_nan_state_gives_nan_dt_and_retires(emu):
+    """A NaN state gives a NaN dt (the reference's cfl*h/max with a NaN max,
+    phrike/solver.py:613-620), not 1e-6, and ph_runs_advance retires that run
+    with t = NaN while the other run advances."""
+    from emu_lib import Emu2D, P
+    from phrike_amd.initial_conditions import khi_2d_periodic
+    e = Emu2D(emu, 16, 16, batch=2)
+    U = np.stack([khi_2d_periodic(16, 16, seed=1), khi_2d_periodic(16, 16, seed=2)])
+    U[1, 0, 3, 5] = np.nan
+    m, d = e.dt(U, 0.4)
+    assert np.isfinite(d[0]) and d[0] > 0 and np.isnan(m[1]) and np.isnan(d[1])
+    t = np.zeros(2)
+    t_end = np.ones(2)
+    active = np.ones(2, dtype=np.int32)
+    steps = np.zeros(2, dtype=np.int32)
+    assert emu.ph_runs_clip(2, P(t), P(t_end), P(d), P(active), None) == 0
+    assert emu.ph_runs_advance(2, P(t), P(t_end), P(d), P(active), P(steps), None) == 0
+    assert active[0] == 1 and active[1] == 0 and np.isnan(t[1]) and t[0] == d[0]
+
+
+def test_device_run_loop_matches_host_loop(emu):
+    """ph_run3d (device-resident dt, clip, advance and stop test) against the
+    reference loop restated on the oracle (phrike/solver.py:1395-1513): same
+    number of steps, same times, same state, with a stop time inside the run
+    and the last step clipped to t_end."""
+    from emu_lib import Emu3D, P
+    from oracle import spectral_np as sn
+    from phrike_amd.initial_conditions import taylor_green_3d
+    n, cfl = 16, 0.4
+    filt = {"enabled": True, "p": 16, "alpha": 36.0}
+    U0 = taylor_green_3d(n, n, n)
+    og = sn.Grid3DOracle(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, filter_params=filt, fft_workers=1)
+    osv = sn.Solver3DOracle(og, sn.Euler3DOracle(1.4), cfl=cfl)
+    t_end = 6.5 * osv.compute_dt(U0)
+    # host loop of the reference
+    U, t, dt, ts = U0, 0.0, osv.compute_dt(U0), []
+    while t < t_end - 1e-12:
+        dt = min(dt, t_end - t)
+        U = osv.step(U, dt)
+        t += dt
+        ts.append(t)
+        dt = osv.compute_dt(U)
+    e = Emu3D(emu, n, n, n)
+    a = np.ascontiguousarray(U0)
+    b = np.zeros_like(a)
+    rs = np.zeros(6)
+    rf = np.zeros(2, dtype=np.int32)
+    rs[2] = t_end
+    rs[3] = ts[2] - 1e-9  # an "output" time between steps 2 and 3: stop after step 3
+    mx = np.zeros(1)
+    assert emu.ph_max_wave_speed3d(e.h, P(a), P(mx), P(rs[1:]), cfl, P(e.ws), None) == 0
+    rf[0] = 1
+    assert emu.ph_run3d(e.h, P(a), P(b), 5, P(rs), P(rf), cfl, 0, P(e.ws), None) == 0
+    # stopped at the stop time, state in Ub (times equal up to the last bit of
+    # the CFL speed: the two states differ by FP64 rounding)
+    assert rf[1] == 3 and rf[0] == 0 and abs(rs[0] - ts[2]) <= 1e-14 * ts[2]
+    rs[3] = np.inf
+    rf[0] = 1
+    assert emu.ph_run3d(e.h, P(b), P(a), 8, P(rs), P(rf), cfl, 1, P(e.ws), None) == 0
+    assert rf[1] == len(ts) == 7 and rf[0] == 0 and abs(rs[0] - t_end) <= 1e-14 * t_end
+    state = b if (rf[1] - 3) % 2 == 0 else a  # even: the second call's Ua (b)
+    assert field_errors(state, U).max() < 1e-12

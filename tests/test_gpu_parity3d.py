@@ -199,14 +199,28 @@ def test_fp32_bulk_vs_fp64(dev):
 
 
 def test_step_rejects_aliasing_and_bad_shapes(dev):
+    """The host layer checks out (shape, dtype, contiguity, no shared memory)
+    before any launch; the C ABI itself rejects outputs that overlap U, fully
+    or in part."""
     import torch
+    from phrike_amd import _lib
     from phrike_amd._lib import HipBackendError
     g, e, s = _setup(16, 16, 16)
     U = torch.ones((5, 16, 16, 16), dtype=torch.float64, device=dev)
-    with pytest.raises(HipBackendError):
+    with pytest.raises(ValueError):
         s.step_into(U, U, 1e-3)
     with pytest.raises(ValueError):
+        s.step_into(U, torch.empty((5, 16, 16, 16), dtype=torch.float32, device=dev), 1e-3)
+    with pytest.raises(ValueError):
+        s.step_into(U, torch.empty((5, 16, 16, 32), dtype=torch.float64, device=dev)[..., ::2], 1e-3)
+    with pytest.raises(ValueError):
         s.step(torch.ones((4, 16, 16, 16), dtype=torch.float64, device=dev), 1e-3)
+    big = torch.ones((6, 16, 16, 16), dtype=torch.float64, device=dev)
+    ws = _lib.ptr(g.workspace())
+    st = _lib.stream_handle(torch, dev)
+    for a, b in ((big[:5], big[:5]), (big[:5], big[1:])):  # full and partial overlap
+        with pytest.raises(HipBackendError, match="overlap"):
+            _lib.call("ph_step3d_ex", s._plan, _lib.ptr(a), _lib.ptr(b), 1e-3, None, 0, ws, st)
 
 
 
