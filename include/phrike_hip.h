@@ -184,6 +184,21 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
               const double* dt_dev, const int* active_dev, int apply_filter, void* ws,
               void* stream);
 
+/* ph_step2d without a workspace: the scratch memory its kernels use (the
+ * spectral buffer of the batch, 8 * nbatch * ny * mxp complex values, mxp =
+ * nx/3 + 1 rounded up to 8 (FP64) or 16 (FP32)) is a static 32 MiB device
+ * array of the library, so the call allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  A batch needing more than 32 MiB is an argument error (nothing
+ * is launched).  Every call on one GPU shares that GPU's array and leaves its
+ * contents behind: calls that overlap on one GPU (other streams, other
+ * threads) race, so serialise them.  The array's device address is resolved
+ * at 2D plan creation on the current device (or at the first call on a
+ * device); arguments and results otherwise as ph_step2d. */
+int ph_step2d_noalloc(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
+                      const double* dt_dev, const int* active_dev, int apply_filter, void* stream);
+
 /* Per-run max of |ux| + sqrt(gamma p / rho) -> max_dev[run] and, when dt_dev
  * is not NULL, dt_dev[run] = cfl*min(dx,dy)/max (1e-6 if max <= 0).  Runs with
  * active_dev[run] == 0 are not read (their outputs are meaningless);

@@ -46,6 +46,8 @@ _SIGNATURES = {
     "ph_deriv2d": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
                           c_void_p, c_void_p]),
+    "ph_step2d_noalloc": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_int,
+                                  c_void_p]),
     "ph_max_wave_speed2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                     c_void_p, c_void_p]),
     "ph_run2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_double, c_void_p,

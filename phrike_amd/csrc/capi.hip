@@ -2,6 +2,7 @@
 // the plan's element type.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -66,6 +67,31 @@ void with2(ph_plan2d_t p, F&& f) {
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Scratch memory of ph_step2d_noalloc: a file-scope device array, so calls
+// into that entry point allocate nothing.  Each GPU the library runs on holds
+// its own copy (at its own address) until the process ends.  In host code the
+// array's name is only a stand-in for the symbol: the device address comes
+// from hipGetSymbolAddress, resolved once per device with that device current
+// (at 2D plan creation, or at the entry point's first call on a device).
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void* scratch_ptr() {
+  int dev = -1;
+  ph::hip_check(hipGetDevice(&dev), "hipGetDevice");
+  need(dev >= 0 && dev < kMaxDevices, "device index out of range for the static scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    ph::hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    need(p != nullptr, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
 }  // namespace
 
 namespace ph {
@@ -236,6 +262,7 @@ int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, in
       p = new ph::Plan2D<float>(nx, ny, lx, ly, dealias != 0, filter_enabled != 0, filter_p,
                                 filter_alpha, gamma, batch);
     *plan = reinterpret_cast<ph_plan2d_t>(p);
+    (void)scratch_ptr();  // set-up of the no-allocation step on this device
   });
 }
 
@@ -288,6 +315,27 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
       need(!ph::ranges_overlap(U, n, out, n), "U and out must not overlap");
       p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev,
              apply_filter != 0, ws, S(stream));
+    });
+  });
+}
+
+int ph_step2d_noalloc(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt, const double* dt_dev,
+                      const int* active_dev, int apply_filter, void* stream) {
+  return guard([&] {
+    need(U && out, "null buffer");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      const size_t n = 4ull * size_t(nbatch) * size_t(p.ncell) * sizeof(R);
+      need(!ph::ranges_overlap(U, n, out, n), "U and out must not overlap");
+      const size_t want = p.step_scratch_bytes(nbatch);
+      if (want > kScratchBytes)
+        throw ph::Error(PH_ERR_ARG, "ph_step2d_noalloc: " + std::to_string(nbatch) + " run(s) of " +
+                                        std::to_string(p.nx) + "x" + std::to_string(p.ny) + " need " +
+                                        std::to_string(want) + " bytes of scratch; the static scratch holds " +
+                                        std::to_string(kScratchBytes) + " (use ph_step2d with a workspace)");
+      p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev, apply_filter != 0,
+             scratch_ptr(), S(stream));
     });
   });
 }

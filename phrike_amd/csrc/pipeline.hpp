@@ -703,6 +703,9 @@ struct Plan2D : PlanBase {
   void check_nb(int nb) const {
     if (nb < 1 || nb > batch) throw Error(-2, "nbatch must be in [1, plan batch]");
   }
+  // workspace bytes step() touches for nb runs: only the spectral buffer A
+  // (the fused step keeps stage states in registers; no reductions)
+  size_t step_scratch_bytes(int nb) const { return 8ull * size_t(nb) * size_t(ny) * size_t(mxp) * sizeof(cx<T>); }
   XFwdArgs<T> xfwd_args(unsigned ikmask, void* ws) const {
     XFwdArgs<T> a{};
     a.in_fs = ncell;

@@ -79,12 +79,16 @@ class Emu2D:
         if rc != 0:
             raise RuntimeError(self.lib.ph_last_error().decode())
 
-    def step(self, U, dt, filt=True, active=None, out=None, dt_dev=None):
+    def step(self, U, dt, filt=True, active=None, out=None, noalloc=False, dt_dev=None):
         U = np.ascontiguousarray(U, dtype=self.dtype)
         out = np.zeros_like(U) if out is None else out
         nb = U.shape[0] if U.ndim == 4 else 1
-        self._chk(self.lib.ph_step2d(self.h, nb, P(U), P(out), dt, P(dt_dev), P(active), int(filt),
-                                     P(self.ws), None))
+        if noalloc:
+            self._chk(self.lib.ph_step2d_noalloc(self.h, nb, P(U), P(out), dt, P(dt_dev), P(active), int(filt),
+                                                 None))
+        else:
+            self._chk(self.lib.ph_step2d(self.h, nb, P(U), P(out), dt, P(dt_dev), P(active), int(filt),
+                                         P(self.ws), None))
         return out
 
     def rhs(self, U):

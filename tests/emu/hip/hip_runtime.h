@@ -20,6 +20,8 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __restrict__
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(X) X
 
 struct dim3 {
   unsigned x, y, z;
@@ -92,5 +94,14 @@ inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
   *ms = 0.f;
+  return hipSuccess;
+}
+inline hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+template <typename T>
+inline hipError_t hipGetSymbolAddress(void** p, const T& symbol) {
+  *p = const_cast<void*>(static_cast<const void*>(&symbol));
   return hipSuccess;
 }

@@ -158,6 +158,23 @@ def test_gll_apply_rejects_in_place(emu):
     np.testing.assert_array_equal(out, F)
 
 
+def test_step2d_noalloc_equals_workspace_step(emu):
+    """ph_step2d_noalloc (static scratch, no workspace argument) gives the same
+    bits as ph_step2d, on two calls in a row with the second smaller; a batch
+    beyond the static scratch is an argument error."""
+    from emu_lib import Emu2D
+    from phrike_amd.initial_conditions import khi_2d_periodic
+    for (n, nb, prec) in ((32, 3, np.float64), (16, 2, np.float32)):
+        e = Emu2D(emu, n, n, batch=nb, dtype=prec)
+        U = np.stack([khi_2d_periodic(n, n, seed=s) for s in range(nb)]).astype(prec)
+        a = e.step(U, 1e-3)
+        b = e.step(U, 1e-3, noalloc=True)
+        assert np.array_equal(a, b)
+    big = Emu2D(emu, 1024, 1024, batch=2, dtype=np.float64)
+    with pytest.raises(RuntimeError, match="static scratch"):
+        big.step(np.ones((2, 4, 1024, 1024)), 1e-3, noalloc=True)
+
+
 def test_hybrid_inactive_run_untouched_with_filter(emu):
     """A masked run (active 0) of a batched hybrid step keeps its output
     buffer bit for bit with the filter on (the nodal-filter apply skips it),
