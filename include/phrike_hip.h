@@ -152,6 +152,33 @@ int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double
  * (phrike/equations.py:622-637).  Deterministic. */
 int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void* ws, void* stream);
 
+/* Monitoring statistics of a state, FP64, out_dev[PH_MON_*] (per run at
+ * out_dev[run * PH_MON_N + f] for ph_monitor2d):
+ *   PH_MON_MASS sum rho, PH_MON_RV2 sum rho |u|^2, PH_MON_VMAX / VMIN max / min
+ *   |u|, PH_MON_RMAX / RMIN max / min rho, PH_MON_RS1 / RS2 sum (rho - c) and
+ *   sum (rho - c)^2 with c = PH_MON_SHIFT (rho of the first cell; mean and
+ *   variance without cancellation), PH_MON_PMAX / PMIN / PSUM max, min, sum
+ *   of p, PH_MON_B2 sum |B|^2 (MHD, else 0).  A NaN cell makes the max / min
+ *   entries NaN.  Deterministic.  The reductions behind
+ *   BaseProblem.compute_velocity_stats / compute_density_stats /
+ *   compute_mhd_stats (phrike/problems/base.py:666-822); the host forms
+ *   v_rms, means and standard deviations from them. */
+#define PH_MON_MASS 0
+#define PH_MON_RV2 1
+#define PH_MON_VMAX 2
+#define PH_MON_VMIN 3
+#define PH_MON_RMAX 4
+#define PH_MON_RMIN 5
+#define PH_MON_RS1 6
+#define PH_MON_RS2 7
+#define PH_MON_PMAX 8
+#define PH_MON_PMIN 9
+#define PH_MON_PSUM 10
+#define PH_MON_B2 11
+#define PH_MON_SHIFT 12
+#define PH_MON_N 13
+int ph_monitor3d(ph_plan3d_t plan, const void* U, double* out_dev, void* ws, void* stream);
+
 /* ------------------------------------------------------------------------
  * Batched 2D Fourier x Fourier grid: `batch` independent runs of one shape,
  * state (batch, 4, ny, nx).  Replaces Grid2D Fourier mode
@@ -218,6 +245,9 @@ int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, doubl
  * (phrike/solver.py:734-757). */
 int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out4_dev, void* ws,
                         void* stream);
+
+/* ph_monitor3d for the first nbatch runs (4-field 2D states). */
+int ph_monitor2d(ph_plan2d_t plan, int nbatch, const void* U, double* out_dev, void* ws, void* stream);
 
 /* Per-run time bookkeeping for device-resident run loops (ensembles): for
  * every run with active_dev[run] != 0,
@@ -329,6 +359,10 @@ int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void
  * SpectralSolverMHD.compute_dt without diffusion (phrike/solver.py:1018-1035). */
 int ph_max_wave_speed_mhd3d(ph_planmhd3d_t plan, const void* U, double* max_dev, double* dt_dev, double cfl,
                             void* ws, void* stream);
+
+/* ph_monitor3d for the 8-field MHD state (PH_MON_B2 = sum |B|^2, p the
+ * thermal pressure). */
+int ph_monitor_mhd3d(ph_planmhd3d_t plan, const void* U, double* out_dev, void* ws, void* stream);
 
 /* FP64 sums to out7_dev[0..6]: mass, momentum x/y/z, energy, magnetic
  * energy, cross helicity.  Replaces MHDEquations3D.conserved_quantities

@@ -19,6 +19,9 @@ PH_STEP_BANDLIMITED = 1
 # device-resident run loop scalars (ph_run3d / ph_run2d)
 PH_RS_T, PH_RS_DT, PH_RS_TEND, PH_RS_TSTOP, PH_RS_DTNEXT, PH_RS_DTUSED, PH_RS_N = range(7)
 PH_RF_ACTIVE, PH_RF_STEPS, PH_RF_N = range(3)
+# monitoring reductions (ph_monitor3d / ph_monitor2d / ph_monitor_mhd3d)
+(PH_MON_MASS, PH_MON_RV2, PH_MON_VMAX, PH_MON_VMIN, PH_MON_RMAX, PH_MON_RMIN, PH_MON_RS1, PH_MON_RS2,
+ PH_MON_PMAX, PH_MON_PMIN, PH_MON_PSUM, PH_MON_B2, PH_MON_SHIFT, PH_MON_N) = range(14)
 
 # symbol -> (restype, argtypes); mirrors include/phrike_hip.h
 _SIGNATURES = {
@@ -38,6 +41,9 @@ _SIGNATURES = {
                          c_void_p]),
     "ph_max_wave_speed3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "ph_conserved_sums3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_monitor3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_monitor2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_monitor_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_plan2d_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,
                                  c_int, c_int, c_double, c_double, c_int]),
     "ph_plan2d_destroy": (c_int, [c_void_p]),

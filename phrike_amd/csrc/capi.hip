@@ -247,6 +247,16 @@ int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void*
   });
 }
 
+int ph_monitor3d(ph_plan3d_t plan, const void* U, double* out_dev, void* ws, void* stream) {
+  return guard([&] {
+    need(U && out_dev && ws, "null buffer");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.template monitor<5>(static_cast<const R*>(U), out_dev, ws, S(stream));
+    });
+  });
+}
+
 int ph_plan2d_create(ph_plan2d_t* plan, int nx, int ny, double lx, double ly, int dtype, int dealias,
                      int filter_enabled, int filter_p, double filter_alpha, double gamma, int batch) {
   return guard([&] {
@@ -387,6 +397,17 @@ int ph_conserved_sums2d(ph_plan2d_t plan, int nbatch, const void* U, double* out
       PH_R2;
       p.check_nb(nbatch);
       p.sums(nbatch, static_cast<const R*>(U), out4_dev, ws, S(stream));
+    });
+  });
+}
+
+int ph_monitor2d(ph_plan2d_t plan, int nbatch, const void* U, double* out_dev, void* ws, void* stream) {
+  return guard([&] {
+    need(U && out_dev && ws, "null buffer");
+    with2(plan, [&](auto& p) {
+      PH_R2;
+      p.check_nb(nbatch);
+      p.monitor(nbatch, static_cast<const R*>(U), out_dev, ws, S(stream));
     });
   });
 }

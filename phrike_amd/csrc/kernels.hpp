@@ -1184,6 +1184,108 @@ __global__ __launch_bounds__(NT) void k_sums_final(const double* part, int nb, d
 }
 
 // ----------------------------------------------------------------------------
+// monitoring reductions (phrike/problems/base.py:490-822: velocity, density
+// and pressure statistics): per run, in FP64,
+//   MON_MASS   sum rho              MON_RV2   sum rho |u|^2
+//   MON_VMAX   max |u|              MON_VMIN  min |u|
+//   MON_RMAX   max rho              MON_RMIN  min rho
+//   MON_RS1    sum (rho - c)        MON_RS2   sum (rho - c)^2   (c = rho of cell 0,
+//              a shift that keeps the variance free of cancellation)
+//   MON_PMAX   max p                MON_PMIN  min p
+//   MON_PSUM   sum p                MON_B2    sum |B|^2 (MHD; 0 otherwise)
+//   MON_SHIFT  c
+// NaN in any cell propagates into the max / min entries.
+// ----------------------------------------------------------------------------
+enum MonField {
+  MON_MASS = 0, MON_RV2, MON_VMAX, MON_VMIN, MON_RMAX, MON_RMIN, MON_RS1, MON_RS2, MON_PMAX, MON_PMIN,
+  MON_PSUM, MON_B2, MON_NRED, MON_SHIFT = MON_NRED, MON_N
+};
+
+__device__ __forceinline__ double dmin_nan(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? a : b;
+}
+__device__ __forceinline__ bool mon_is_max(int f) { return f == MON_VMAX || f == MON_RMAX || f == MON_PMAX; }
+__device__ __forceinline__ bool mon_is_min(int f) { return f == MON_VMIN || f == MON_RMIN || f == MON_PMIN; }
+__device__ __forceinline__ double mon_init(int f) {
+  return mon_is_max(f) ? -__builtin_inf() : (mon_is_min(f) ? __builtin_inf() : 0.0);
+}
+__device__ __forceinline__ double mon_join(int f, double a, double b) {
+  return mon_is_max(f) ? dmax_nan(a, b) : (mon_is_min(f) ? dmin_nan(a, b) : a + b);
+}
+
+// NCOMP: 4 (2D Euler), 5 (3D Euler), 8 (3D ideal MHD)
+template <typename T, int NCOMP>
+__global__ __launch_bounds__(NT) void k_monitor_partial(const T* u, long long fs, long long ss, long long n,
+                                                        T gamma, double* part) {
+  __shared__ double red[MON_NRED][NT];
+  const T* us = u + (long long)blockIdx.y * ss;
+  const double c = double(us[0]);
+  double acc[MON_NRED];
+#pragma unroll
+  for (int f = 0; f < MON_NRED; ++f) acc[f] = mon_init(f);
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+    const T rho = us[i];
+    const T ux = us[fs + i] / rho, uy = us[2 * fs + i] / rho;
+    const T uz = NCOMP >= 5 ? us[3 * fs + i] / rho : T(0);
+    const T q2 = NCOMP >= 5 ? (ux * ux + uy * uy) + uz * uz : ux * ux + uy * uy;
+    const T E = us[(NCOMP == 4 ? 3 : 4) * fs + i];
+    T b2 = T(0);
+    if constexpr (NCOMP == 8) {
+      const T bx = us[5 * fs + i], by = us[6 * fs + i], bz = us[7 * fs + i];
+      b2 = (bx * bx + by * by) + bz * bz;
+    }
+    const T p = NCOMP == 8 ? (gamma - T(1)) * ((E - (T(0.5) * rho) * q2) - T(0.5) * b2)
+                           : (gamma - T(1)) * (E - (T(0.5) * rho) * q2);
+    const double v = sqrt(double(q2)), r = double(rho), dr = r - c;
+    const double val[MON_NRED] = {r, r * double(q2), v, v, r, r, dr, dr * dr, double(p), double(p), double(p),
+                                  double(b2)};
+#pragma unroll
+    for (int f = 0; f < MON_NRED; ++f) acc[f] = mon_join(f, acc[f], val[f]);
+  }
+#pragma unroll
+  for (int f = 0; f < MON_NRED; ++f) red[f][threadIdx.x] = acc[f];
+  lds_barrier();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int f = 0; f < MON_NRED; ++f) red[f][threadIdx.x] = mon_join(f, red[f][threadIdx.x], red[f][threadIdx.x + w]);
+    }
+    lds_barrier();
+  }
+  if (threadIdx.x < MON_NRED)
+    part[((long long)blockIdx.y * gridDim.x + blockIdx.x) * MON_NRED + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// one block per run: reduce nb partials -> out[run * MON_N + f]
+template <typename T>
+__global__ __launch_bounds__(NT) void k_monitor_final(const double* part, int nb, const T* u, long long ss,
+                                                      double* out) {
+  __shared__ double red[MON_NRED][NT];
+  const double* ps = part + (long long)blockIdx.x * nb * MON_NRED;
+  double acc[MON_NRED];
+#pragma unroll
+  for (int f = 0; f < MON_NRED; ++f) acc[f] = mon_init(f);
+  for (int i = threadIdx.x; i < nb; i += NT) {
+#pragma unroll
+    for (int f = 0; f < MON_NRED; ++f) acc[f] = mon_join(f, acc[f], ps[(long long)i * MON_NRED + f]);
+  }
+#pragma unroll
+  for (int f = 0; f < MON_NRED; ++f) red[f][threadIdx.x] = acc[f];
+  lds_barrier();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int f = 0; f < MON_NRED; ++f) red[f][threadIdx.x] = mon_join(f, red[f][threadIdx.x], red[f][threadIdx.x + w]);
+    }
+    lds_barrier();
+  }
+  if (threadIdx.x < MON_NRED) out[(long long)blockIdx.x * MON_N + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x == 0) out[(long long)blockIdx.x * MON_N + MON_SHIFT] = double(u[(long long)blockIdx.x * ss]);
+}
+
+// ----------------------------------------------------------------------------
 // per-run time bookkeeping of the run loop (phrike/solver.py:764-797):
 //   dt = min(dt, t_end - t) before a step; t += dt and
 //   active = t < t_end - 1e-12 after it.  FP64 throughout.

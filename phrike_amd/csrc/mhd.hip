@@ -199,7 +199,7 @@ struct PlanMHD3D : PlanBase {
     offAcc = al(offPrj + prj.wsBytes);
     offUs = al(offAcc + st);
     offPart = al(offUs + st);
-    wsBytes = al(offPart + sizeof(double) * NPART * 7);
+    wsBytes = al(offPart + sizeof(double) * NPART * MON_NRED);
   }
   ~PlanMHD3D() override {
     for (CE* p : {kxc, nyqc, kyv, kzv})
@@ -306,6 +306,14 @@ struct PlanMHD3D : PlanBase {
     check_launch("k_mhd_sums");
     hipLaunchKernelGGL((k_sums_final<7>), dim3(1), dim3(NT), 0, st, Part(ws), nb, out);
     check_launch("k_sums_final");
+  }
+  void monitor(const T* U, double* out, void* ws, hipStream_t st) const {
+    const int nb = blocks(ncell);
+    hipLaunchKernelGGL((k_monitor_partial<T, 8>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma),
+                       Part(ws));
+    check_launch("k_monitor_partial(mhd)");
+    hipLaunchKernelGGL((k_monitor_final<T>), dim3(1), dim3(NT), 0, st, (const double*)Part(ws), nb, U, 0ll, out);
+    check_launch("k_monitor_final(mhd)");
   }
 };
 
@@ -416,6 +424,16 @@ int ph_max_wave_speed_mhd3d(ph_planmhd3d_t plan, const void* U, double* max_dev,
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;
       p.max_speed(static_cast<const R*>(U), max_dev, dt_dev, cfl, ws, MS(stream));
+    });
+  });
+}
+
+int ph_monitor_mhd3d(ph_planmhd3d_t plan, const void* U, double* out_dev, void* ws, void* stream) {
+  return mguard([&] {
+    if (!U || !ws || !out_dev) throw ph::Error(PH_ERR_ARG, "null buffer");
+    withm(plan, [&](auto& p) {
+      using R = Elem<decltype(p)>;
+      p.monitor(static_cast<const R*>(U), out_dev, ws, MS(stream));
     });
   });
 }

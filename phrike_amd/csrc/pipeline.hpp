@@ -332,7 +332,7 @@ struct Plan3D : PlanBase {
     offB = al(offA + szA);
     offUs = offB;  // unused: the fused step keeps stage states in registers
     offPart = al(offB + szB);
-    wsBytes = al(offPart + sizeof(double) * NPART * 5);
+    wsBytes = al(offPart + sizeof(double) * NPART * MON_NRED);
   }
   ~Plan3D() override {
     if (xcol_sm1) (void)hipFree(xcol_sm1);
@@ -650,6 +650,17 @@ struct Plan3D : PlanBase {
     hipLaunchKernelGGL((k_sums_final<5>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out);
     check_launch("k_sums_final");
   }
+  // monitoring statistics (MON_* fields) of a state with NCOMP fields
+  template <int NCOMP>
+  void monitor(const T* U, double* out, void* ws, hipStream_t st) const {
+    double* part = Part(ws);
+    const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    hipLaunchKernelGGL((k_monitor_partial<T, NCOMP>), dim3(nb, 1), dim3(NT), 0, st, U, ncell, 0ll, ncell, T(gamma),
+                       part);
+    check_launch("k_monitor_partial");
+    hipLaunchKernelGGL((k_monitor_final<T>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, U, 0ll, out);
+    check_launch("k_monitor_final");
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -690,7 +701,7 @@ struct Plan2D : PlanBase {
     offA = 0;
     offUs = al(8ull * batch * ny * mxp * cs);
     offPart = al(offUs + 4ull * batch * ncell * sizeof(T));
-    wsBytes = al(offPart + sizeof(double) * (size_t)NPART * 5 * batch);
+    wsBytes = al(offPart + sizeof(double) * (size_t)NPART * MON_NRED * batch);
   }
   ~Plan2D() override {
     dx.release();
@@ -856,6 +867,15 @@ struct Plan2D : PlanBase {
     check_launch("k_sums_partial(2d)");
     hipLaunchKernelGGL((k_sums_final<4>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, out);
     check_launch("k_sums_final(2d)");
+  }
+  void monitor(int nb, const T* U, double* out, void* ws, hipStream_t st) const {
+    double* part = Part(ws);
+    const int npb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
+    hipLaunchKernelGGL((k_monitor_partial<T, 4>), dim3(npb, nb), dim3(NT), 0, st, U, ncell, 4 * ncell, ncell,
+                       T(gamma), part);
+    check_launch("k_monitor_partial(2d)");
+    hipLaunchKernelGGL((k_monitor_final<T>), dim3(nb), dim3(NT), 0, st, (const double*)part, npb, U, 4 * ncell, out);
+    check_launch("k_monitor_final(2d)");
   }
 };
 

@@ -31,8 +31,16 @@ composition the closed-form checks cannot pin are recorded:
       bc="wall"; phrike/grid.py:144-738, phrike/solver.py:33-131): one RHS
       and 1 / 5 RK4 steps (clamp, filter, wall rule) with the filter on and
       off: the x-uniform restriction of the hybrid Fourier-x / LGL-y path.
+* F11 snapshots written by the reference's save_solution_snapshot
+      (phrike/io.py:103-236) for 3D Euler, 2D Euler and 3D MHD states (meta
+      stored as a pickled dict), for the restart loader's tests.
+* F12 BaseProblem's monitoring statistics (phrike/problems/base.py:490-822)
+      on torch tensors: conservation errors, velocity, density and MHD
+      statistics of 3D Euler, 2D Euler and 3D MHD states.
 
-The outputs are plain ``.npz`` data (no pickles).  numba, pyfftw and finufft
+The outputs are plain ``.npz`` data (no pickles), except F11, whose ``meta``
+entry is the reference's own pickled dict (the case the loader has to
+handle; nothing here unpickles it).  numba, pyfftw and finufft
 are absent here, so the reference runs its plain numpy/scipy branches; the
 versions used are recorded in every file.
 """
@@ -338,13 +346,93 @@ def f10(grid_m, eq_m, sol_m, out):
     print("F10", flush=True)
 
 
+def f11(grid_m, eq_m, sol_m, out):
+    """Snapshots written by the reference's own save_solution_snapshot
+    (phrike/io.py:103-236) for a 3D, a 2D and a 3D MHD state: their meta is a
+    pickled dict, which the hip backend's loader must not unpickle."""
+    import shutil
+    import tempfile
+    io_m = __import__("importlib").import_module("phrike.io")
+    d = os.path.join(out, "ref_snapshots")
+    os.makedirs(d, exist_ok=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        g3 = grid_m.Grid3D(8, 8, 16, 2 * np.pi, np.pi, 4 * np.pi, dealias=True, fft_workers=1)
+        e3 = eq_m.EulerEquations3D(gamma=1.4)
+        U3 = ics.smooth_random_state_3d(8, 8, 16, seed=3, kmax=2)
+        f = io_m.save_solution_snapshot(tmp, 0.125, U3, g3, e3)
+        shutil.copy(f, os.path.join(d, "euler3d_" + os.path.basename(f)))
+        g2 = grid_m.Grid2D(16, 8, 1.0, 0.5, dealias=True, fft_workers=1, basis_x="fourier", basis_y="fourier")
+        e2 = eq_m.EulerEquations2D(gamma=5.0 / 3.0)
+        U2 = ics.khi_2d_periodic(16, 8, seed=2)
+        f = io_m.save_solution_snapshot(tmp, 0.5, U2, g2, e2, tag="khi")
+        shutil.copy(f, os.path.join(d, "euler2d_" + os.path.basename(f)))
+        gm = grid_m.Grid3D(8, 8, 8, 1.0, 1.0, 1.0, dealias=True, fft_workers=1)
+        em = eq_m.MHDEquations3D(gamma=5.0 / 3.0)
+        X, _, _ = gm.xyz_mesh()
+        Um = _alfven_ic(em, X)
+        f = io_m.save_solution_snapshot(tmp, 0.25, Um, gm, em)
+        shutil.copy(f, os.path.join(d, "mhd3d_" + os.path.basename(f)))
+    print("F11", sorted(os.listdir(d)), flush=True)
+
+
+def f12(grid_m, eq_m, sol_m, out):
+    """The monitoring statistics of BaseProblem (phrike/problems/base.py:
+    490-822) on torch CPU tensors (the branch a tensor state takes):
+    conservation errors, velocity, density and MHD statistics of a 3D Euler,
+    a 2D Euler and a 3D MHD state."""
+    import importlib
+    import types as _t
+
+    import torch
+    base = importlib.import_module("phrike.problems.base")
+    BP = base.BaseProblem
+    res = {}
+
+    def stats(tag, solver, U0, U1):
+        me = _t.SimpleNamespace(monitoring_initial_values=None)
+        me.monitoring_initial_values = solver.equations.conserved_quantities(torch.as_tensor(U0))
+        T1 = torch.as_tensor(U1)
+        out = {}
+        out.update(BP.compute_conservation_errors(me, solver, T1))
+        out.update(BP.compute_velocity_stats(me, solver, T1))
+        out.update(BP.compute_density_stats(me, solver, T1))
+        out.update(BP.compute_mhd_stats(me, solver, T1))
+        res[f"{tag}_U0"], res[f"{tag}_U1"] = U0, U1
+        res[f"{tag}_keys"] = np.array(list(out.keys()))
+        res[f"{tag}_vals"] = np.array([float(v) for v in out.values()])
+
+    rng = np.random.default_rng(12)
+    e3 = eq_m.EulerEquations3D(gamma=1.4)
+    U0 = ics.smooth_random_state_3d(16, 8, 32, seed=31, kmax=3)
+    U1 = U0 * (1.0 + 0.01 * rng.standard_normal(U0.shape))
+    stats("e3", _t.SimpleNamespace(equations=e3, grid=None), U0, U1)
+    e2 = eq_m.EulerEquations2D(gamma=1.4)
+    V0 = ics.khi_2d_periodic(32, 16, seed=8)
+    V1 = V0 * (1.0 + 0.01 * rng.standard_normal(V0.shape))
+    stats("e2", _t.SimpleNamespace(equations=e2, grid=_t.SimpleNamespace()), V0, V1)
+    em = eq_m.MHDEquations3D(gamma=5.0 / 3.0)
+    H = ics.smooth_random_state_3d(8, 8, 8, seed=41, kmax=2)
+    B = np.stack([1.0 + 0.2 * H[1] / H[0], 0.2 * H[2] / H[0], 0.2 * H[3] / H[0]])
+    M0 = np.concatenate([H, B])
+    M0[4] = H[4] + 0.5 * (B ** 2).sum(axis=0)
+    M1 = M0 * (1.0 + 0.01 * rng.standard_normal(M0.shape))
+    # max|divB| comes from the solver; a stand-in returning a fixed value
+    stats("mhd", _t.SimpleNamespace(equations=em, grid=None, max_div_b=lambda U: 0.125), M0, M1)
+    np.savez_compressed(os.path.join(out, "monitor.npz"), **res, **_meta())
+    print("F12", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-long", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     grid_m, eq_m, sol_m, leg_m = _load_reference()
-    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6", "f9", "f10"}
+    only = set(a.only.split(",")) if a.only else {"f1", "f3", "f5", "f6", "f9", "f10", "f11", "f12"}
+    if "f12" in only:
+        f12(grid_m, eq_m, sol_m, HERE)
+    if "f11" in only:
+        f11(grid_m, eq_m, sol_m, HERE)
     if "f10" in only:
         f10(grid_m, eq_m, sol_m, HERE)
     if "f3" in only:
