@@ -152,6 +152,19 @@ int ph_max_wave_speed3d(ph_plan3d_t plan, const void* U, double* max_dev, double
  * (phrike/equations.py:622-637).  Deterministic. */
 int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void* ws, void* stream);
 
+/* 3D transforms of a plan created with dealias = 0 and filter_enabled = 0
+ * (every mode kept): ph_fft3d writes the unnormalised forward transform of
+ * nfields (1..5) real fields as half spectra (nfields, nz, ny, nx/2 + 1) of
+ * interleaved complex values; ph_ifft3d takes such half spectra, read as the
+ * kx >= 0 half of a Hermitian spectrum, and writes the real fields of the
+ * normalised inverse.  The reference's Grid3D.fftn / ifftn (phrike/grid.py:
+ * 1497-1513, full complex spectra; ifftn returns the real part) are these
+ * with the kx < 0 half filled by Hermitian symmetry (fftn) or the input
+ * first reduced to its Hermitian part, whose inverse is that real part
+ * (ifftn); the host layer does that bookkeeping. */
+int ph_fft3d(ph_plan3d_t plan, int nfields, const void* f, void* spec, void* ws, void* stream);
+int ph_ifft3d(ph_plan3d_t plan, int nfields, const void* spec, void* f, void* ws, void* stream);
+
 /* Monitoring statistics of a state, FP64, out_dev[PH_MON_*] (per run at
  * out_dev[run * PH_MON_N + f] for ph_monitor2d):
  *   PH_MON_MASS sum rho, PH_MON_RV2 sum rho |u|^2, PH_MON_VMAX / VMIN max / min

@@ -41,6 +41,8 @@ _SIGNATURES = {
                          c_void_p]),
     "ph_max_wave_speed3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "ph_conserved_sums3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_fft3d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ph_ifft3d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_monitor3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_monitor2d": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_monitor_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -247,6 +247,28 @@ int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void*
   });
 }
 
+int ph_fft3d(ph_plan3d_t plan, int nfields, const void* f, void* spec, void* ws, void* stream) {
+  return guard([&] {
+    need(f && spec && ws, "null buffer");
+    need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.fft3d(nfields, static_cast<const R*>(f), static_cast<R*>(spec), ws, S(stream));
+    });
+  });
+}
+
+int ph_ifft3d(ph_plan3d_t plan, int nfields, const void* spec, void* f, void* ws, void* stream) {
+  return guard([&] {
+    need(f && spec && ws, "null buffer");
+    need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
+    with3(plan, [&](auto& p) {
+      using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
+      p.ifft3d(nfields, static_cast<const R*>(spec), static_cast<R*>(f), ws, S(stream));
+    });
+  });
+}
+
 int ph_monitor3d(ph_plan3d_t plan, const void* U, double* out_dev, void* ws, void* stream) {
   return guard([&] {
     need(U && out_dev && ws, "null buffer");

@@ -208,7 +208,11 @@ struct LdsSize {
   static constexpr int value = (PADB > 0 ? N + (N >> PADB) : N) * LS;
 };
 
-template <typename T, int N, int L, int DIR, int R, int NS, int LS = L, int PADB = 0>
+// TWS: stride of the twiddle table (a table of exp(-2 pi i m / (TWS N))
+// entries serves a length-N transform with TWS = 2: the single-row x kernels
+// share one table between their N/2-point transforms and the real-to-complex
+// split)
+template <typename T, int N, int L, int DIR, int R, int NS, int LS = L, int PADB = 0, int TWS = 1>
 __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int b, cx<T>* v) {
   const int l = b % L, j = b / L;
 #pragma unroll
@@ -217,7 +221,7 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
     const int k = j % NS;
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-      cx<T> w = tw[r * k * (N / (NS * R))];
+      cx<T> w = tw[TWS * (r * k * (N / (NS * R)))];
       if constexpr (DIR > 0) w.y = -w.y;
       v[r] = cmul(v[r], w);
     }
@@ -228,7 +232,7 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
 // one Stockham pass, LDS -> LDS (in place; ends with a barrier).  Lines are
 // interleaved with stride LS >= L (padding keeps 16-byte accesses free of
 // bank conflicts when L is not a multiple of 8).
-template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0, class PL = Plan<N>>
+template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1>
 __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
   constexpr int R = PL::radix(P);
   constexpr int NS = PassNs<N, P, PL>::value;
@@ -238,7 +242,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS, PADB>(buf, tw, b, v[q]);
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS, PADB, TWS>(buf, tw, b, v[q]);
   }
   lds_barrier();
 #pragma unroll
@@ -256,20 +260,20 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 }
 
 template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L, int PADB = 0,
-          class PL = Plan<N>>
+          class PL = Plan<N>, int TWS = 1>
 __device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
   if constexpr (P < PEND) {
-    pass_lds<T, N, L, DIR, NT, P, LS, PADB, PL>(buf, tw, tid);
-    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB, PL>(buf, tw, tid);
+    pass_lds<T, N, L, DIR, NT, P, LS, PADB, PL, TWS>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB, PL, TWS>(buf, tw, tid);
   }
 }
 
 // Full transform of the L lines in buf (line stride LS), result in buf
 // (natural order).  Caller must have synchronised after filling buf; ends
 // with a barrier.
-template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0, class PL = Plan<N>>
+template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1>
 __device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  passes_lds_range<T, N, L, DIR, NT, 0, PL::npass, LS, PADB, PL>(buf, tw, tid);
+  passes_lds_range<T, N, L, DIR, NT, 0, PL::npass, LS, PADB, PL, TWS>(buf, tw, tid);
 }
 
 // Full transform with the last pass left in registers v[q][r]; output r of

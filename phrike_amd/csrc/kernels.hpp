@@ -139,6 +139,14 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #define PH_X_WAVES_PER_EU 1
 #endif
 
+// Single-row x kernels (1, default): a block transforms one row y of every
+// field; the real row is packed as z[p] = x[2p] + i x[2p+1] and transformed
+// with an NX/2-point complex FFT.  0: two rows per block (z = row0 + i row1,
+// NX-point transforms).
+#ifndef PH_XROW
+#define PH_XROW 1
+#endif
+
 template <typename T, int NX, bool DA>
 struct XGeo {
   static constexpr int NT = XThreads<T, NX>::value;
@@ -183,6 +191,7 @@ __device__ __forceinline__ T xquot(T a, T b, T rb) {
   else return a * rb;
 }
 
+#if !PH_XROW
 // Kept half spectra of rows y0, y0+1 of NCI fields, loaded into registers in
 // 16-byte chunks; unsplit() writes z = row0 + i*row1 in natural order to LDS
 // (line l = field l), with zeros in the band MX..NX-MX.
@@ -290,6 +299,8 @@ __device__ __forceinline__ void x_split_store(const cx<T>* buf, cx<T>* dst, long
   }
 }
 
+#endif  // !PH_XROW
+
 // ----------------------------------------------------------------------------
 // x forward
 // ----------------------------------------------------------------------------
@@ -323,6 +334,7 @@ template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
 
+#if !PH_XROW
 // Fluxes (or copies) of the conserved values q (both rows of the thread's x
 // positions) -> forward x transform of two rows at once -> kept modes to
 // a.out.  Shared by k_xfwd and the fused k_xmid.
@@ -437,6 +449,8 @@ __device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const
   }
 }
 
+#endif  // !PH_XROW
+
 template <typename T, int NX, bool DA>
 __device__ __forceinline__ void x_tables(const T* sig, const T* kk, T* s_sig, T* s_k, int tid) {
   using G = XGeo<T, NX, DA>;
@@ -446,6 +460,7 @@ __device__ __forceinline__ void x_tables(const T* sig, const T* kk, T* s_sig, T*
   }
 }
 
+#if !PH_XROW
 template <typename T, int NX, bool DA, int MODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xfwd(XFwdArgs<T> a) {
   using G = XGeo<T, NX, DA>;
@@ -476,6 +491,8 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   }
   x_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y0, q);
 }
+
+#endif  // !PH_XROW
 
 // ----------------------------------------------------------------------------
 // column passes (y and z): tiles of TM consecutive mx columns, lines along the
@@ -840,6 +857,7 @@ __device__ __forceinline__ void rk_inputs(const XInvArgs<T>& a, long long o, T (
   }
 }
 
+#if !PH_XROW
 template <typename T, int NX, bool DA, int NC, int MODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xinv(XInvArgs<T> a) {
   using G = XGeo<T, NX, DA>;
@@ -918,6 +936,8 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   }
 }
 
+#endif  // !PH_XROW
+
 // Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
 // the fluxes of the stage s+1 state and their forward x transform, or (stage
 // 4) the new state's forward x transform for the post-step filter, which
@@ -933,6 +953,7 @@ struct XMidArgs {
   XFwdArgs<T> fwd_last;  // stage 4 (copy mode, 5 or 4 fields)
 };
 
+#if !PH_XROW
 template <typename T, int NX, bool DA, int NC, int FMODE>
 __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xmid(XMidArgs<T> m) {
   using G = XGeo<T, NX, DA>;
@@ -1033,6 +1054,655 @@ __global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k
   } else {
     x_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y0, q);
   }
+}
+
+#endif  // !PH_XROW
+
+#if PH_XROW
+// ----------------------------------------------------------------------------
+// Single-row x kernels.  A block owns row y of one slab for every field.  The
+// real row x[0..NX) is packed as z[p] = x[2p] + i x[2p+1] (p < M = NX/2) and
+// transformed with an M-point complex FFT in LDS.  With E, O the spectra of
+// the even and odd samples, Z = E + i O, so the half spectrum is
+//   X[k] = (Z[k] + Z*[M-k])/2 - i W^k (Z[k] - Z*[M-k])/2,  W = exp(-2 pi i/NX),
+// k = 0..M (indices mod M), and the inverse packs the kept modes back as
+//   Z'[k] = (X[k] + X*[M-k]) + i W^-k (X[k] - X*[M-k]),
+//   z[p] = sum_k Z'[k] exp(+2 pi i k p/M)   (x[2p] = Re z, x[2p+1] = Im z),
+// taking X[0] and X[M] real (the real part of the full inverse, as the
+// reference's ifft(...).real does).  Thread t owns the packed points
+// p = t + j*NT (j < PJ), i.e. the x pairs (2p, 2p+1), which it loads and
+// stores as one complex value (16 bytes in FP64).  Blocks are small (NX/2
+// FP64, NX/4 FP32 threads), so several share a CU and the load, transform
+// and store phases of different blocks overlap.
+// ----------------------------------------------------------------------------
+template <typename T, int NX>
+struct XRThreads {
+  static constexpr int value = sizeof(T) == 8 ? clampi(NX / 2, 64, 512) : clampi(NX / 4, 64, 256);
+};
+
+template <typename T, int NX, bool DA>
+struct XRGeo {
+  static constexpr int NT = XRThreads<T, NX>::value;
+  static constexpr int M = NX / 2;                         // packed points of a row
+  static constexpr int MX = DA ? NX / 3 + 1 : NX / 2 + 1;  // kept x modes
+  static constexpr int K = ColElem<T>::PW;                 // modes per 16-byte chunk
+  static constexpr int MC = (MX + K - 1) / K;              // chunks holding kept modes
+  static constexpr bool PALL = M >= NT;                    // every thread owns points
+  static constexpr int PJ = PALL ? M / NT : 1;             // packed points per thread
+  // FP32 threads own two adjacent points (x = 4t..4t+3, one 16-byte access);
+  // otherwise point p = t + j*NT
+  static constexpr bool CONSEC = sizeof(T) == 4 && PJ == 2;
+  static __device__ __forceinline__ int point(int tid, int j) { return CONSEC ? 2 * tid + j : tid + j * NT; }
+};
+
+// the thread's x pairs of one row (row: element 0 of the row): loaded /
+// stored as PJ complex values, one 16-byte access per thread in FP32
+template <typename G, typename T>
+__device__ __forceinline__ void xr_load_pairs(const T* row, int tid, cx<T> (&v)[G::PJ]) {
+  if constexpr (G::CONSEC) {
+    const pair4f q = reinterpret_cast<const pair4f*>(row)[tid];
+    v[0] = mk(q.a, q.b);
+    v[1] = mk(q.c, q.d);
+  } else {
+#pragma unroll
+    for (int j = 0; j < G::PJ; ++j) {
+      const int p = G::point(tid, j);
+      v[j] = (G::PALL || p < G::M) ? reinterpret_cast<const cx<T>*>(row)[p] : mk(T(0), T(0));
+    }
+  }
+}
+template <typename G, typename T>
+__device__ __forceinline__ void xr_store_pairs(T* row, int tid, const cx<T> (&v)[G::PJ]) {
+  if constexpr (G::CONSEC) {
+    pair4f q;
+    q.a = v[0].x;
+    q.b = v[0].y;
+    q.c = v[1].x;
+    q.d = v[1].y;
+    reinterpret_cast<pair4f*>(row)[tid] = q;
+  } else {
+#pragma unroll
+    for (int j = 0; j < G::PJ; ++j) {
+      const int p = G::point(tid, j);
+      if (G::PALL || p < G::M) reinterpret_cast<cx<T>*>(row)[p] = v[j];
+    }
+  }
+}
+// LDS of a packed row: M transform points and the Nyquist slot M
+template <int NX, int LINES>
+constexpr int xrbuf_size() { return LdsSize<NX / 2 + 1, xstride(LINES), PH_XPADB>::value; }
+
+template <typename T, int NX, bool DA>
+__device__ __forceinline__ void xr_tables(const T* sig, const T* kk, T* s_sig, T* s_k, int tid) {
+  using G = XRGeo<T, NX, DA>;
+  for (int i = tid; i < G::MX; i += G::NT) {
+    s_sig[i] = sig[i];
+    s_k[i] = kk[i];
+  }
+}
+
+// Kept modes of one row of NCI fields, loaded into registers in 16-byte
+// chunks; place() writes them to LDS at their natural positions (line l =
+// field l) with zeros in the band MX..M.
+template <typename T, int NX, bool DA, int NCI>
+struct XRSpec {
+  using G = XRGeo<T, NX, DA>;
+  using C = typename ChunkT<T, G::K>::type;
+  static constexpr int NTASK = NCI * G::MC;
+  static constexpr int PER = (NTASK + G::NT - 1) / G::NT;
+  C a[PER];
+
+  __device__ __forceinline__ void load(const cx<T>* src, long long fs, int nfl, int tid) {
+    const C* s = reinterpret_cast<const C*>(src);
+    const long long fsc = fs / G::K;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = tid + q * G::NT;
+      a[q] = C{};
+      if (NTASK % G::NT == 0 || t < NTASK) {
+        const int l = t / G::MC, m = t - l * G::MC;
+        if (l < nfl) a[q] = s[l * fsc + m];
+      }
+    }
+  }
+
+  __device__ __forceinline__ void place(cx<T>* buf, int tid) const {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int t = tid + q * G::NT;
+      if (NTASK % G::NT == 0 || t < NTASK) {
+        const int l = t / G::MC, m = t - l * G::MC;
+#pragma unroll
+        for (int i = 0; i < G::K; ++i) {
+          const int k = m * G::K + i;
+          if (k < G::MX) buf[xslot<NCI>(k, l)] = ch_get(a[q], i);
+        }
+      }
+    }
+    constexpr int ZB = G::M + 1 - G::MX;
+    if constexpr (ZB > 0) {
+      for (int t = tid; t < NCI * ZB; t += G::NT) {
+        const int l = t / ZB, k = G::MX + (t - l * ZB);
+        buf[xslot<NCI>(k, l)] = mk(T(0), T(0));
+      }
+    }
+  }
+};
+
+// natural-order half spectra in LDS (positions 0..M) -> packed spectrum Z'
+// (positions 0..M-1), in place: the task of pair (k, M-k) reads and writes
+// only those two slots
+template <typename T, int NX, int LINES, int NT>
+__device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
+  constexpr int M = NX / 2, H = M / 2 + 1;
+  for (int t = tid; t < LINES * H; t += NT) {
+    const int l = t / H, k = t - l * H, k2 = M - k;
+    cx<T> a = buf[xslot<LINES>(k, l)], b = buf[xslot<LINES>(k2, l)];
+    if (k == 0) {  // X[0], X[M]: real parts
+      a.y = T(0);
+      b.y = T(0);
+    }
+    const cx<T> bc = mk(b.x, -b.y);
+    const cx<T> d1 = cmul(mk(tw[k].x, -tw[k].y), a - bc);  // W^-k (X[k] - X*[M-k])
+    buf[xslot<LINES>(k, l)] = (a + bc) + mk(-d1.y, d1.x);
+    if (k != 0 && k2 != k) {
+      const cx<T> ac = mk(a.x, -a.y);
+      const cx<T> d2 = cmul(mk(tw[k2].x, -tw[k2].y), b - ac);
+      buf[xslot<LINES>(k2, l)] = (b + ac) + mk(-d2.y, d2.x);
+    }
+  }
+}
+
+// Z (M-point forward transforms of LX packed lines, in LDS) -> kept modes
+// X[k] = (E + W^k O) * sig[k] (* i k[k]) of lines l < nl, stored as 16-byte
+// chunks to dst (one row of the line's field; field stride fs)
+template <typename T, int NX, bool DA, int LX>
+__device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw, cx<T>* dst, long long fs, int nl,
+                                               const T* sig, const T* kk, bool ik, int tid) {
+  using G = XRGeo<T, NX, DA>;
+  using C = typename ChunkT<T, G::K>::type;
+  constexpr int NTASK = LX * G::MC, M = G::M;
+  C* d = reinterpret_cast<C*>(dst);
+  const long long fsc = fs / G::K;
+  for (int t = tid; t < NTASK; t += G::NT) {
+    const int l = t / G::MC, m = t - l * G::MC;
+    if (l < nl) {
+      C c{};
+#pragma unroll
+      for (int i = 0; i < G::K; ++i) {
+        const int k = m * G::K + i;
+        if (k < G::MX) {
+          const cx<T> a = buf[xslot<LX>(k & (M - 1), l)];
+          const cx<T> z2 = buf[xslot<LX>((M - k) & (M - 1), l)];
+          const cx<T> e = mk(T(0.5) * (a.x + z2.x), T(0.5) * (a.y - z2.y));
+          const cx<T> h = mk(T(0.5) * (a.x - z2.x), T(0.5) * (a.y + z2.y));  // (Z[k] - Z*[M-k]) / 2
+          cx<T> X = e + cmul(tw[k], mk(h.y, -h.x));                           // E + W^k (-i h)
+          X = cscale(X, sig[k]);
+          if (ik) X = mul_ik(X, kk[k]);
+          ch_set(c, i, X);
+        }
+      }
+      d[l * fsc + m] = c;
+    }
+  }
+}
+
+// Fluxes (or copies) of the conserved values q at the thread's x pairs ->
+// forward transform of the row -> kept modes to a.out.  Shared by k_xfwd and
+// k_xmid.
+template <typename T, int NX, bool DA, int MODE>
+__device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
+                                           const T* sig, const T* kk, int tid, int slab, int y,
+                                           const T (&q)[XRGeo<T, NX, DA>::PJ][2][XFwdShape<MODE>::NC]) {
+  using G = XRGeo<T, NX, DA>;
+  using S = XFwdShape<MODE>;
+  constexpr int LX = S::LX, NC = S::NC, NT = G::NT, PJ = G::PJ, M = G::M;
+  // primitive recovery (phrike/equations.py:104-117, 150-164; MHD 922-933)
+  constexpr int NPR = MODE == XF_MHD3 ? 11 : 6;
+  T pr[PJ][2][NPR];
+  if constexpr (MODE == XF_MHD3) {
+    // rho ux uy uz p* Bx By Bz E uB (E + p*)  (phrike/equations.py:922-933, 949-1000)
+#pragma unroll
+    for (int j = 0; j < PJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const T* u = q[j][r];
+        const T rho = u[0];
+        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
+        const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
+        const T Bx = u[5], By = u[6], Bz = u[7];
+        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+        const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+        const T p = (a.gamma - T(1)) * ((u[4] - kin) - mag);
+        const T ps = p + mag;
+        T* w = pr[j][r];
+        w[0] = rho; w[1] = ux; w[2] = uy; w[3] = uz; w[4] = ps;
+        w[5] = Bx; w[6] = By; w[7] = Bz; w[8] = u[4];
+        w[9] = (ux * Bx + uy * By) + uz * Bz;
+        w[10] = u[4] + ps;
+      }
+  } else if constexpr (MODE != XF_COPY) {
+#pragma unroll
+    for (int j = 0; j < PJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const T* u = q[j][r];
+        const T rho = u[0];
+        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
+        if constexpr (MODE == XF_FLUX3) {
+          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
+          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+          const T p = (a.gamma - T(1)) * (u[4] - kin);
+          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = uz;
+          pr[j][r][4] = p;
+          pr[j][r][5] = u[4] + p;  // E + p
+        } else {
+          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri);
+          const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+          const T p = (a.gamma - T(1)) * (u[3] - kin);
+          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = T(0);
+          pr[j][r][4] = p;
+          pr[j][r][5] = u[3] + p;
+        }
+      }
+  }
+  cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y * a.mxp;
+  const int nl = MODE == XF_COPY ? ncomp : LX;
+#pragma unroll 1
+  for (int g = 0; g < S::NG; ++g) {
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      const int p = G::point(tid, j);
+      if (G::PALL || p < M) {
+#pragma unroll
+        for (int l = 0; l < LX; ++l) {
+          T v[2];
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            if constexpr (MODE == XF_COPY) {
+              v[r] = q[j][r][l];
+            } else if constexpr (MODE == XF_MHD3) {
+              // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000);
+              // the direction is picked by selects (a register index by the
+              // run-time g would put pr in scratch memory)
+              const T* w = pr[j][r];
+              const T ug = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
+              const T Bg = g == 0 ? w[5] : (g == 1 ? w[6] : w[7]);
+              const T mg = w[0] * ug;
+              if (l == 0) v[r] = mg;
+              else if (l <= 3) {
+                v[r] = mg * w[l];
+                if (l == g + 1) v[r] = v[r] + w[4];
+                v[r] = v[r] - Bg * w[4 + l];
+              } else if (l == 4) v[r] = w[10] * ug - Bg * w[9];
+              else v[r] = (l - 5 == g) ? T(0) : ug * w[l] - w[l - 4] * Bg;
+            } else {
+              // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
+              const T* w = pr[j][r];
+              const T ud = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
+              const T md = w[0] * ud;
+              if (l == 0) v[r] = md;
+              else if (l == NC - 1) v[r] = w[5] * ud;
+              else {
+                v[r] = md * w[l];
+                if (l == g + 1) v[r] = v[r] + w[4];
+              }
+            }
+          }
+          buf[xslot<LX>(p, l)] = mk(v[0], v[1]);
+        }
+      }
+    }
+    lds_barrier();
+    fft_lds<T, M, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
+    xr_split_store<T, NX, DA, LX>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
+                                  (a.ikmask >> (g * LX)) & 1u, tid);
+    lds_barrier();
+  }
+}
+
+template <typename T, int NX, bool DA, int MODE>
+__global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xfwd(XFwdArgs<T> a) {
+  using G = XRGeo<T, NX, DA>;
+  using S = XFwdShape<MODE>;
+  constexpr int NT = G::NT, NC = S::NC, PJ = G::PJ;
+  PH_XTW_DECL(T, NX);
+  __shared__ cx<T> buf[xrbuf_size<NX, S::LX>()];
+  __shared__ T s_sig[G::MX], s_k[G::MX];
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (gated_off(a.active, a.act_all, slab)) return;
+  const int y = blockIdx.x;
+  PH_XTW_FILL(a.tw, NX, NT, tid);
+  xr_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
+  const T* src = a.in + (long long)slab * a.in_ss + (long long)y * NX;
+  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
+  T q[PJ][2][NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    cx<T> v[PJ];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) v[j] = mk(T(0), T(0));
+    if (c < ncomp) xr_load_pairs<G>(src + (long long)c * a.in_fs, tid, v);
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      q[j][0][c] = v[j].x;
+      q[j][1][c] = v[j].y;
+    }
+  }
+  xr_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y, q);
+}
+
+// RK4 stage combine of the x pair (2p, 2p+1) with k_s = -v (phrike/solver.py:
+// 1220-1230 3D, 502-513 2D); as rk_point: stages 1-3 give the next stage
+// state in u and the updated accumulator in acc_new, stage 4 gives
+// acc + dt/6 k4; the 2D form clamps u (phrike/solver.py:467-488)
+template <typename T, int NC, bool CLAMP>
+__device__ __forceinline__ void rk_pair(const XInvArgs<T>& a, const T (&v)[2][NC], double dtd, const cx<T> (&un)[NC],
+                                        const cx<T> (&acc)[NC], T (&u)[2][NC], cx<T> (&acc_new)[NC]) {
+  // one assignment per output (selects, no branches), which keeps the
+  // arrays in registers
+  const T half = T(0.5 * dtd), full = T(dtd), sixth = T(dtd / 6.0), third = T(dtd / 3.0);
+  const bool last = a.stage >= 4;
+  const T coef = last ? sixth : (a.stage == 3 ? full : half);
+  const T w = (a.stage == 1) ? sixth : third;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const T k0 = -v[0][c], k1 = -v[1][c];
+    const cx<T> bu = last ? acc[c] : un[c];
+    u[0][c] = bu.x + coef * k0;
+    u[1][c] = bu.y + coef * k1;
+    const cx<T> ba = a.stage == 1 ? un[c] : acc[c];
+    acc_new[c] = last ? mk(u[0][c], u[1][c]) : mk(ba.x + w * k0, ba.y + w * k1);
+  }
+  if constexpr (CLAMP) {
+    clamp2d(u[0], a.gamma, a.rho_min, a.p_min);
+    clamp2d(u[1], a.gamma, a.rho_min, a.p_min);
+  }
+}
+
+// the row's inverse transform of NC fields into buf (natural packed order)
+template <typename T, int NX, bool DA, int NC>
+__device__ __forceinline__ void xr_inverse(const XRSpec<T, NX, DA, NC>& sp, cx<T>* buf, const cx<T>* tw, int tid) {
+  using G = XRGeo<T, NX, DA>;
+  sp.place(buf, tid);
+  lds_barrier();  // also covers the block's twiddle fill
+  xr_pack<T, NX, NC, G::NT>(buf, tw, tid);
+  lds_barrier();
+  fft_lds<T, G::M, NC, +1, G::NT, xstride(NC), PH_XPADB, XPlan<G::M>, 2>(buf, tw, tid);
+}
+
+template <typename T, int NX, bool DA, int NC, int MODE>
+__global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xinv(XInvArgs<T> a) {
+  using G = XRGeo<T, NX, DA>;
+  constexpr int NT = G::NT, PJ = G::PJ;
+  PH_XTW_DECL(T, NX);
+  __shared__ cx<T> buf[xrbuf_size<NX, NC>()];
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (gated_off(a.active, a.act_all, slab)) return;
+  const int y = blockIdx.x;
+  PH_XTW_FILL(a.tw, NX, NT, tid);
+  const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
+  static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
+  static_assert(MODE != XI_MHDSTAGE || NC == 8, "the MHD stage combine takes the 8 MHD fields");
+  XRSpec<T, NX, DA, NC> sp;
+  sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, nfl, tid);
+  xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
+  const long long base = (long long)slab * a.ss + (long long)y * NX;
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
+  // the transformed values at the thread's x pairs: v[j][r][c] at x = 2p + r
+  T v[PJ][2][NC];
+#pragma unroll
+  for (int j = 0; j < PJ; ++j) {
+    const int p = G::point(tid, j);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+      v[j][0][c] = z.x;
+      v[j][1][c] = z.y;
+    }
+  }
+  auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+  if constexpr (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= nfl) continue;
+      cx<T> o[PJ];
+      if constexpr (MODE == XI_ADD) xr_load_pairs<G>(row(a.out, c), tid, o);
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        const T s0 = MODE == XI_RHS ? -v[j][0][c] : v[j][0][c];
+        const T s1 = MODE == XI_RHS ? -v[j][1][c] : v[j][1][c];
+        o[j] = MODE == XI_ADD ? mk(o[j].x + s0, o[j].y + s1) : mk(s0, s1);
+      }
+      xr_store_pairs<G>(row(a.out, c), tid, o);
+    }
+  } else if constexpr (MODE == XI_PROJE || MODE == XI_MHDSTAGE) {
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      const int p = G::point(tid, j);
+      if (!G::PALL && p >= G::M) continue;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const long long o = base + 2 * p + r;
+        if constexpr (MODE == XI_PROJE) {
+          // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933
+          T* u = a.out;
+          const T rho = u[o];
+          const T ux = u[a.fs + o] / rho, uy = u[2 * a.fs + o] / rho, uz = u[3 * a.fs + o] / rho;
+          const T b0 = u[5 * a.fs + o], b1 = u[6 * a.fs + o], b2 = u[7 * a.fs + o];
+          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+          const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
+          const T pp = (a.gamma - T(1)) * ((u[4 * a.fs + o] - kin) - mag);
+          const T Bx = v[j][r][0], By = v[j][r][1], Bz = v[j][r][2];
+          u[4 * a.fs + o] = (pp / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+          u[5 * a.fs + o] = Bx;
+          u[6 * a.fs + o] = By;
+          u[7 * a.fs + o] = Bz;
+        } else {
+          T u[8];
+          mhd_stage_point<T>(a, v[j][r], o, dtd, u);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) a.us[c * a.fs + o] = u[c];
+        }
+      }
+    }
+  } else {
+    cx<T> un[NC][PJ], acc[NC][PJ];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        un[c][j] = mk(T(0), T(0));
+        acc[c][j] = mk(T(0), T(0));
+      }
+      if (a.stage < 4) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+      if (a.stage > 1) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+    }
+    cx<T> an[NC][PJ], us[NC][PJ];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      cx<T> un_j[NC], acc_j[NC], an_j[NC];
+      T u[2][NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        un_j[c] = un[c][j];
+        acc_j[c] = acc[c][j];
+      }
+      rk_pair<T, NC, MODE == XI_STAGE2>(a, v[j], dtd, un_j, acc_j, u, an_j);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        an[c][j] = an_j[c];
+        us[c][j] = mk(u[0][c], u[1][c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (a.stage < 4) {
+        xr_store_pairs<G>(row(a.out, c), tid, an[c]);
+        xr_store_pairs<G>(row(a.us, c), tid, us[c]);
+      } else {
+        xr_store_pairs<G>(row(a.out, c), tid, us[c]);
+      }
+    }
+  }
+}
+
+// Fused stage boundary (as the two-row form above): x inverse + RK combine
+// of stage s, then the fluxes of the stage s+1 state and their forward x
+// transform (stages 1-3), or the new state's forward x transform for the
+// post-step filter (stage 4).  The block loads its spectral row completely
+// before it writes that row.
+template <typename T, int NX, bool DA, int NC, int FMODE>
+__global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xmid(XMidArgs<T> m) {
+  using G = XRGeo<T, NX, DA>;
+  constexpr int NT = G::NT, PJ = G::PJ;
+  static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
+  PH_XTW_DECL(T, NX);
+  constexpr int LB0 = NC > XFwdShape<FMODE>::LX ? NC : XFwdShape<FMODE>::LX;
+  constexpr int LB = LB0 > XFwdShape<XF_COPY>::LX ? LB0 : XFwdShape<XF_COPY>::LX;
+  __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
+  __shared__ T s_sig[G::MX], s_k[G::MX];
+  const XInvArgs<T>& a = m.inv;
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (gated_off(a.active, a.act_all, slab)) return;
+  const int y = blockIdx.x;
+  XRSpec<T, NX, DA, NC> sp;
+  sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, NC, tid);
+  PH_XTW_FILL(a.tw, NX, NT, tid);
+  const XFwdArgs<T>& fa = a.stage < 4 ? m.fwd : m.fwd_last;
+  xr_tables<T, NX, DA>(fa.sig, fa.kk, s_sig, s_k, tid);
+  const long long base = (long long)slab * a.ss + (long long)y * NX;
+  // the RK combine's inputs (x pairs), per field: U (PH_XMID_PREFETCH 2), or
+  // U and the accumulator (1), loaded before the inverse transform so their
+  // latency overlaps it; the rest after it
+  constexpr bool PRE_UN = PH_XMID_PREFETCH != 0, PRE_ACC = PH_XMID_PREFETCH == 1;
+  auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+  cx<T> un[NC][PJ], acc[NC][PJ];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      un[c][j] = mk(T(0), T(0));
+      acc[c][j] = mk(T(0), T(0));
+    }
+  if constexpr (PRE_UN) {
+    if (a.stage < 4) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+    }
+  }
+  if constexpr (PRE_ACC) {
+    if (a.stage > 1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+    }
+  }
+  xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
+  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
+  if constexpr (!PRE_UN) {
+    if (a.stage < 4) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+    }
+  }
+  if constexpr (!PRE_ACC) {
+    if (a.stage > 1) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+    }
+  }
+  T q[PJ][2][5];
+  cx<T> an[NC][PJ];
+#pragma unroll
+  for (int j = 0; j < PJ; ++j) {
+    const int p = G::point(tid, j);
+    T v[2][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+      v[0][c] = z.x;
+      v[1][c] = z.y;
+    }
+    T u[2][NC];
+    cx<T> un_j[NC], acc_j[NC], an_j[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      un_j[c] = un[c][j];
+      acc_j[c] = acc[c][j];
+    }
+    rk_pair<T, NC, NC == 4>(a, v, dtd, un_j, acc_j, u, an_j);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) an[c][j] = an_j[c];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) q[j][r][c] = c < NC ? u[r][c] : T(0);
+  }
+  if (a.stage < 4 || a.keep_state) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, an[c]);
+  }
+  lds_barrier();  // buf is reused by the forward transform
+  if (a.stage < 4) {
+    T qf[PJ][2][XFwdShape<FMODE>::NC];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
+    xr_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, qf);
+  } else {
+    xr_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
+  }
+}
+#endif  // PH_XROW
+
+// launch shape of the x kernels: threads per block and rows per block
+template <typename T, int NX>
+struct XLaunch {
+#if PH_XROW
+  static constexpr int threads = XRThreads<T, NX>::value, rows = 1;
+#else
+  static constexpr int threads = XThreads<T, NX>::value, rows = 2;
+#endif
+};
+
+// ----------------------------------------------------------------------------
+// Half spectra of a plan that keeps every mode, between its spectral buffer B
+// ([field][kz][ky][mxp], FP32 rows pair-planar) and a plain complex array
+// (field, kz, ky, kx < mx) interleaved re/im; the copy in scales by `scale`
+// and zeroes B's padding modes mx..mxp-1 (the column passes read them).
+// ----------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ long long spec_index(long long row, int m, int mxp, bool im) {
+  // position (in T elements) of mode m of a spectral row in B
+  if constexpr (sizeof(T) == 8) return (row * mxp + m) * 2 + (im ? 1 : 0);
+  else return row * mxp * 2 + 4 * (m >> 1) + (im ? 2 : 0) + (m & 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spec_out(const T* B, T* out, long long rows, int mx, int mxp) {
+  const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  if (i >= rows * mx) return;
+  const long long row = i / mx;
+  const int m = int(i - row * mx);
+  out[2 * i] = B[spec_index<T>(row, m, mxp, false)];
+  out[2 * i + 1] = B[spec_index<T>(row, m, mxp, true)];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_spec_in(const T* in, T* B, long long rows, int mx, int mxp, T scale) {
+  const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  if (i >= rows * mxp) return;
+  const long long row = i / mxp;
+  const int m = int(i - row * mxp);
+  const long long o = row * mx + m;
+  B[spec_index<T>(row, m, mxp, false)] = m < mx ? in[2 * o] * scale : T(0);
+  B[spec_index<T>(row, m, mxp, true)] = m < mx ? in[2 * o + 1] * scale : T(0);
 }
 
 // ----------------------------------------------------------------------------

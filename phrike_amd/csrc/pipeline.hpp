@@ -152,14 +152,13 @@ inline bool supported_len(int n) { return n >= 8 && n <= 1024 && is_pow2(n); }
 // and N/2 + 1
 template <typename T, int N, int MODE>
 void launch_xfwd_n(int mx, dim3 grid, const XFwdArgs<T>& a, hipStream_t st) {
-  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xfwd<T, N, true, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
-  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xfwd<T, N, false, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xfwd<T, N, true, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xfwd<T, N, false, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
   else throw Error(-2, "unsupported x mode count");
 }
 template <typename T, int MODE>
 void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t st) {
-  dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, (launch_xfwd_n<T, N_, MODE>(a.mx, grid, a, st)));
+  PH_SWITCH_N(nx, (launch_xfwd_n<T, N_, MODE>(a.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
 template <typename T, int MODE>
@@ -173,26 +172,24 @@ void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
 
 template <typename T, int N, int NC, int MODE>
 void launch_xinv_n(int mx, dim3 grid, const XInvArgs<T>& a, hipStream_t st) {
-  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xinv<T, N, true, NC, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
-  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xinv<T, N, false, NC, MODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xinv<T, N, true, NC, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xinv<T, N, false, NC, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
   else throw Error(-2, "unsupported x mode count");
 }
 template <typename T, int NC, int MODE>
 void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t st) {
-  dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, (launch_xinv_n<T, N_, NC, MODE>(a.mx, grid, a, st)));
+  PH_SWITCH_N(nx, (launch_xinv_n<T, N_, NC, MODE>(a.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
 template <typename T, int N, int NC, int FMODE>
 void launch_xmid_n(int mx, dim3 grid, const XMidArgs<T>& a, hipStream_t st) {
-  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xmid<T, N, true, NC, FMODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
-  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xmid<T, N, false, NC, FMODE>), grid, dim3(XThreads<T, N>::value), 0, st, a);
+  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xmid<T, N, true, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xmid<T, N, false, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
   else throw Error(-2, "unsupported x mode count");
 }
 template <typename T, int NC, int FMODE>
 void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
-  dim3 grid(nrows / 2, nslab);
-  PH_SWITCH_N(nx, (launch_xmid_n<T, N_, NC, FMODE>(a.inv.mx, grid, a, st)));
+  PH_SWITCH_N(nx, (launch_xmid_n<T, N_, NC, FMODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
@@ -287,6 +284,8 @@ struct Plan3D : PlanBase {
   int mx, mxp;
   long long ncell;
   CE* xcol_sm1 = nullptr;  // sigma_x - 1 per column of the column passes
+  T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
+  bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
   // workspace layout (bytes)
   size_t offA, offB, offUs, offPart, wsBytes;
   static constexpr int NPART = 1024;
@@ -324,6 +323,12 @@ struct Plan3D : PlanBase {
       hip_check(hipMalloc(&xcol_sm1, sizeof(CE) * ncol), "hipMalloc(column sigma-1)");
       hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncol, hipMemcpyHostToDevice), "copy column sigma-1");
     }
+    {
+      std::vector<T> ones(mx, T(1));
+      hip_check(hipMalloc(&unit_x, sizeof(T) * mx), "hipMalloc(unit x table)");
+      hip_check(hipMemcpy(unit_x, ones.data(), sizeof(T) * mx, hipMemcpyHostToDevice), "copy unit x table");
+    }
+    full = !dealias && !filt;
     const size_t cs = sizeof(cx<T>);
     const size_t szA = (size_t)na * nz * ny * mxp * cs;
     const size_t szB = (size_t)nb * nz * ay.nr * mxp * cs;
@@ -336,6 +341,7 @@ struct Plan3D : PlanBase {
   }
   ~Plan3D() override {
     if (xcol_sm1) (void)hipFree(xcol_sm1);
+    if (unit_x) (void)hipFree(unit_x);
     dx.release();
     dy.release();
     dz.release();
@@ -628,6 +634,38 @@ struct Plan3D : PlanBase {
       launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
     }
     check_launch("k_xinv");
+  }
+
+  // Unnormalised forward transform of nf real fields to their half spectra
+  // (nf, nz, ny, nx/2 + 1), interleaved complex; the inverse takes such half
+  // spectra (Hermitian), scales by 1/(nx ny nz) and writes the real fields.
+  // Only for a plan that keeps every mode and has no filter.
+  void fft3d(int nf, const T* f, T* out, void* ws, hipStream_t st) const {
+    if (!full || az.nr != nz || ay.nr != ny) throw Error(-2, "fft3d needs a plan without dealiasing or filter");
+    XFwdArgs<T> a = xfwd_args(f, nf, 0u, ws);
+    a.sig = unit_x;
+    launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
+    check_launch("k_xfwd(fft3d)");
+    yfwd(false, nf, 0, ws, st);
+    zline(true, nf, ws, st);
+    const long long rows = (long long)nf * nz * ny;
+    hipLaunchKernelGGL((k_spec_out<T>), dim3((unsigned)((rows * mx + NT - 1) / NT)), dim3(NT), 0, st,
+                       reinterpret_cast<const T*>(B(ws)), out, rows, mx, mxp);
+    check_launch("k_spec_out");
+  }
+  void ifft3d(int nf, const T* in, T* f, void* ws, hipStream_t st) const {
+    if (!full || az.nr != nz || ay.nr != ny) throw Error(-2, "ifft3d needs a plan without dealiasing or filter");
+    const long long rows = (long long)nf * nz * ny;
+    hipLaunchKernelGGL((k_spec_in<T>), dim3((unsigned)((rows * mxp + NT - 1) / NT)), dim3(NT), 0, st, in,
+                       reinterpret_cast<T*>(B(ws)), rows, mx, mxp, T(1.0 / (double(nx) * double(ny) * double(nz))));
+    check_launch("k_spec_in");
+    zline(false, nf, ws, st);
+    yinv(nf, ws, st);
+    XInvArgs<T> a = xinv_args(ws);
+    a.nf = nf;
+    a.out = f;
+    launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+    check_launch("k_xinv(ifft3d)");
   }
 
   void max_speed(const T* U, double* out, double* dt_out, double cfl, void* ws, hipStream_t st,

@@ -127,6 +127,8 @@ class Grid3D:
             lib = _lib.load()
             for h in getattr(self, "_plans", {}).values():
                 lib.ph_plan3d_destroy(h)
+            if getattr(self, "_fplan", None) is not None:
+                lib.ph_plan3d_destroy(self._fplan)
         except Exception:
             pass
 
@@ -169,6 +171,88 @@ class Grid3D:
     def xyz_mesh(self):
         Z, Y, X = torch.meshgrid(self.z, self.y, self.x, indexing="ij")
         return X, Y, Z
+
+    # ---- spectral helpers of the reference's Grid3D (phrike/grid.py:1396-1401,
+    # 1497-1518) ----
+    @property
+    def k2(self):
+        """kx^2 + ky^2 + kz^2 on (Nz, Ny, Nx) (phrike/grid.py:1396-1400)."""
+        return (self.kx[None, None, :] ** 2 + self.ky[None, :, None] ** 2) + self.kz[:, None, None] ** 2
+
+    @property
+    def k2_safe(self):
+        """k2 with its zero (the mean mode) replaced by 1 (phrike/grid.py:1401)."""
+        k2 = self.k2
+        return torch.where(k2 == 0.0, torch.ones_like(k2), k2)
+
+    def _apply_masks(self, F):
+        """F *= dealias mask; F *= filter sigma, in place (phrike/grid.py:
+        1515-1518); the tensor-product factors are applied axis by axis."""
+        fon, fp, fa = self._filter
+        for axis, n in ((-1, self.Nx), (-2, self.Ny), (-3, self.Nz)):
+            f = _mask_1d(n, self.dealias) * _sigma_1d(n, fon, fp, fa)
+            shape = [1, 1, 1]
+            shape[axis] = n
+            F *= torch.as_tensor(f, dtype=F.real.dtype, device=F.device).reshape(shape)
+        return F
+
+    def _full_plan(self):
+        """A plan that keeps every mode and has no filter (for fftn/ifftn)."""
+        if getattr(self, "_fplan", None) is None:
+            h = _lib.c_void_p()
+            with torch.cuda.device(self.device):
+                _lib.call("ph_plan3d_create", _lib.ctypes.byref(h), self.Nx, self.Ny, self.Nz, self.Lx, self.Ly,
+                          self.Lz, self._dtype_code, 0, 0, 8, 36.0, self._gamma)
+            n = _lib.c_size_t()
+            _lib.call("ph_plan3d_workspace_bytes", h, _lib.ctypes.byref(n))
+            self._fplan = h
+            self._fws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+        return self._fplan, self._fws
+
+    def _ctype(self):
+        return torch.complex64 if self.dtype == torch.float32 else torch.complex128
+
+    def fftn(self, f):
+        """Complex FFT over the last three axes (phrike/grid.py:1497-1504):
+        the library's forward transform gives the kx >= 0 half; the rest
+        follows from Hermitian symmetry (the fields are real)."""
+        f = self._as_fields(f)
+        lead = f.shape[:-3]
+        flat = f.reshape(-1, self.Nz, self.Ny, self.Nx)
+        mx = self.Nx // 2 + 1
+        half = torch.empty((flat.shape[0], self.Nz, self.Ny, mx), dtype=self._ctype(), device=self.device)
+        plan, ws = self._full_plan()
+        st = _lib.stream_handle(torch, self.device)
+        for s in range(0, flat.shape[0], 5):
+            e = min(s + 5, flat.shape[0])
+            _lib.call("ph_fft3d", plan, e - s, _lib.ptr(flat[s:e]), _lib.ptr(half[s:e]), _lib.ptr(ws), st)
+        full = torch.empty((flat.shape[0], self.Nz, self.Ny, self.Nx), dtype=half.dtype, device=self.device)
+        full[..., :mx] = half
+        if self.Nx > 2:
+            # F[kz, ky, kx] = conj(F[-kz, -ky, -kx]) for kx > Nx/2
+            neg = torch.roll(torch.flip(half[..., 1:self.Nx - mx + 1], dims=(-3, -2)), shifts=(1, 1), dims=(-3, -2))
+            full[..., mx:] = torch.flip(neg, dims=(-1,)).conj()
+        return full.reshape(*lead, self.Nz, self.Ny, self.Nx)
+
+    def ifftn(self, F):
+        """Real part of the inverse complex FFT over the last three axes
+        (phrike/grid.py:1506-1513): that is the inverse of the Hermitian part
+        (F(k) + conj F(-k))/2, whose kx >= 0 half the library transforms."""
+        F = torch.as_tensor(F, device=self.device).to(self._ctype())
+        if F.shape[-3:] != (self.Nz, self.Ny, self.Nx):
+            raise ValueError(f"expected (..., {self.Nz}, {self.Ny}, {self.Nx}), got {tuple(F.shape)}")
+        lead = F.shape[:-3]
+        flat = F.reshape(-1, self.Nz, self.Ny, self.Nx)
+        mx = self.Nx // 2 + 1
+        rev = torch.roll(torch.flip(flat, dims=(-3, -2, -1)), shifts=(1, 1, 1), dims=(-3, -2, -1))  # F(-k)
+        herm = (0.5 * (flat + rev.conj()))[..., :mx].contiguous()
+        out = torch.empty((flat.shape[0], self.Nz, self.Ny, self.Nx), dtype=self.dtype, device=self.device)
+        plan, ws = self._full_plan()
+        st = _lib.stream_handle(torch, self.device)
+        for s in range(0, flat.shape[0], 5):
+            e = min(s + 5, flat.shape[0])
+            _lib.call("ph_ifft3d", plan, e - s, _lib.ptr(herm[s:e]), _lib.ptr(out[s:e]), _lib.ptr(ws), st)
+        return out.reshape(*lead, self.Nz, self.Ny, self.Nx)
 
     # full 3D tables, materialised on request only (phrike/grid.py:1404-1415)
     @property

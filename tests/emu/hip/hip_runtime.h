@@ -1,16 +1,20 @@
 // CPU emulation of the small HIP subset the backend's kernels and host code
 // use, for running the real kernel sources on the host in tests (bounds and
 // index bugs show up as wrong numbers or sanitizer reports, never as GPU
-// faults).  Each block runs its threads as std::threads sharing one
-// std::barrier (__syncthreads); blocks run one after another, so __shared__
-// becomes a function-local static.  Test infrastructure only.
+// faults).  The threads of a block run as fibers (ucontext) on the calling
+// OS thread: each runs until its next __syncthreads or its end, and a round
+// over all live fibers completes the barrier (threads that return early drop
+// out).  Blocks run one after another, so __shared__ becomes a
+// function-local static.  Test infrastructure only.
 #pragma once
-#include <barrier>
+#include <ucontext.h>
+
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
-#include <thread>
+#include <functional>
+#include <memory>
 #include <vector>
 
 #define __global__
@@ -29,30 +33,58 @@ struct dim3 {
 };
 
 namespace emu {
-inline thread_local dim3 tIdx, bIdx, bDim, gDim;
-inline thread_local std::barrier<>* bar = nullptr;
+inline dim3 tIdx, bIdx, bDim, gDim;
+
+struct Fiber {
+  ucontext_t ctx;
+  std::unique_ptr<char[]> stack;
+  bool done = false;
+};
+inline ucontext_t sched_ctx;
+inline std::vector<Fiber> fibers;
+inline int cur = 0;
+inline std::function<void()>* body = nullptr;
+constexpr size_t kStack = size_t(256) << 10;
+
+inline void trampoline() {
+  (*body)();
+  fibers[cur].done = true;  // returns to sched_ctx through uc_link
+}
+inline void sync() { swapcontext(&fibers[cur].ctx, &sched_ctx); }
 
 template <typename K, typename... A>
 void launch(K kernel, dim3 grid, dim3 block, A... args) {
   const unsigned nt = block.x * block.y * block.z;
-  for (unsigned bz = 0; bz < grid.z; ++bz)
-    for (unsigned by = 0; by < grid.y; ++by)
-      for (unsigned bx = 0; bx < grid.x; ++bx) {
-        std::barrier<> b(nt);
-        std::vector<std::thread> th;
-        th.reserve(nt);
-        for (unsigned t = 0; t < nt; ++t)
-          th.emplace_back([&, t] {
-            tIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
-            bIdx = dim3(bx, by, bz);
-            bDim = block;
-            gDim = grid;
-            bar = &b;
-            kernel(args...);
-            b.arrive_and_drop();
-          });
-        for (auto& x : th) x.join();
+  const unsigned nblocks = grid.x * grid.y * grid.z;
+  if (fibers.size() < nt) fibers.resize(nt);
+  std::function<void()> f = [&] { kernel(args...); };
+  body = &f;
+  bDim = block;
+  gDim = grid;
+  for (unsigned bi = 0; bi < nblocks; ++bi) {
+    bIdx = dim3(bi % grid.x, (bi / grid.x) % grid.y, bi / (grid.x * grid.y));
+    for (unsigned t = 0; t < nt; ++t) {
+      Fiber& fb = fibers[t];
+      if (!fb.stack) fb.stack.reset(new char[kStack]);
+      getcontext(&fb.ctx);
+      fb.ctx.uc_stack.ss_sp = fb.stack.get();
+      fb.ctx.uc_stack.ss_size = kStack;
+      fb.ctx.uc_link = &sched_ctx;
+      makecontext(&fb.ctx, trampoline, 0);
+      fb.done = false;
+    }
+    // rounds: every live fiber runs to its next barrier (or its end)
+    for (bool live = true; live;) {
+      live = false;
+      for (unsigned t = 0; t < nt; ++t) {
+        if (fibers[t].done) continue;
+        cur = int(t);
+        tIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
+        swapcontext(&sched_ctx, &fibers[t].ctx);
+        live = live || !fibers[t].done;
       }
+    }
+  }
 }
 }  // namespace emu
 
@@ -60,7 +92,7 @@ void launch(K kernel, dim3 grid, dim3 block, A... args) {
 #define blockIdx (::emu::bIdx)
 #define blockDim (::emu::bDim)
 #define gridDim (::emu::gDim)
-inline void __syncthreads() { ::emu::bar->arrive_and_wait(); }
+inline void __syncthreads() { ::emu::sync(); }
 #define __builtin_amdgcn_s_barrier() __syncthreads()
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))

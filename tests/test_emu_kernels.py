@@ -256,3 +256,24 @@ def test_device_run_loop_matches_host_loop(emu):
     assert rf[1] == len(ts) == 7 and rf[0] == 0 and abs(rs[0] - t_end) <= 1e-14 * t_end
     state = b if (rf[1] - 3) % 2 == 0 else a  # even: the second call's Ua (b)
     assert field_errors(state, U).max() < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+def test_fft3d_half_spectrum(emu, dtype, tol):
+    """ph_fft3d / ph_ifft3d of a plan that keeps every mode: the half spectra
+    equal numpy's rfftn (kx >= 0) and the inverse returns the fields."""
+    from emu_lib import Emu3D, P
+    nx, ny, nz = 16, 8, 8
+    e = Emu3D(emu, nx, ny, nz, dtype=dtype, dealias=False, filt=(0, 8, 36.0))
+    rng = np.random.default_rng(4)
+    f = np.ascontiguousarray(rng.standard_normal((2, nz, ny, nx)).astype(dtype))
+    spec = np.zeros((2, nz, ny, nx // 2 + 1), dtype=np.complex128 if dtype == np.float64 else np.complex64)
+    assert emu.ph_fft3d(e.h, 2, P(f), P(spec), P(e.ws), None) == 0
+    ref = np.fft.rfftn(f.astype(np.float64), axes=(-3, -2, -1))
+    assert np.abs(spec - ref).max() <= tol * np.abs(ref).max()
+    back = np.zeros_like(f)
+    assert emu.ph_ifft3d(e.h, 2, P(spec), P(back), P(e.ws), None) == 0
+    assert np.abs(back - f).max() <= tol * np.abs(f).max()
+    # a plan with the 2/3 rule refuses
+    d = Emu3D(emu, nx, ny, nz, dtype=dtype)
+    assert emu.ph_fft3d(d.h, 2, P(f), P(spec), P(d.ws), None) != 0

@@ -43,3 +43,22 @@ def test_eager_equals_graph(dev):
     b = _server({32: 2}, use_graph=False).serve(reqs, realtime=False)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x.state, y.state)
+
+
+def test_pool_mode_resizes_and_matches_fixed(dev):
+    """Memory-managed buckets (hbm_budget): buckets grow from zero slots as
+    requests queue, carry their active runs across a resize and shrink when
+    idle; every result equals the fixed-capacity server's bit for bit."""
+    from phrike_amd.serving import EnsembleServer, RunRequest, slot_bytes
+    reqs = [RunRequest(i, 32 if i % 2 else 64, 20 + i, 0.004 + 0.002 * i, arrival=0.03 * i) for i in range(8)]
+    fixed = _server({32: 4, 64: 4}).serve(reqs, realtime=False)
+    budget = 3 * slot_bytes(64, "double") + 2 * slot_bytes(32, "double")
+    srv = EnsembleServer(hbm_budget=budget, sizes=(32, 64), precision="double", steps_per_replay=6,
+                         keep_states=True, max_slots=4)
+    pool = srv.serve(reqs, realtime=True)
+    assert [r.rid for r in pool] == list(range(8))
+    assert srv.peak_bytes <= budget
+    assert sum(b.resizes for b in srv.buckets.values()) >= 2
+    for a, b in zip(fixed, pool):
+        assert a.steps == b.steps and a.t_final == b.t_final
+        np.testing.assert_array_equal(a.state, b.state)
