@@ -575,6 +575,50 @@ def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, n, prec, c
             "command": command, "kernels": kern}
 
 
+def run_loop_timing(s, U, steps):
+    """The device run loop (ph_run3d) over `steps` steps with dt from the
+    CFL condition after every step, as the reference's run loop does
+    (phrike/solver.py:1395-1513): ms per step, against the fixed-dt step and
+    the standalone max wave-speed reduction (one read of the state) that the
+    fused per-row maxima of the step's last kernel replace."""
+    import torch
+    from phrike_amd import _lib
+    plan, ws, st = s._ctx()
+    dev = U.device
+    rs = torch.zeros(_lib.PH_RS_N, dtype=torch.float64, device=dev)
+    rf = torch.zeros(_lib.PH_RF_N, dtype=torch.int32, device=dev)
+    rs[_lib.PH_RS_TEND] = 1e30
+    rs[_lib.PH_RS_TSTOP] = float("inf")
+    a, b = U.clone(), torch.empty_like(U)
+    _lib.call("ph_max_wave_speed3d", plan, _lib.ptr(a), None, _lib.ptr(rs[_lib.PH_RS_DT:]), s.cfl, ws, st)
+
+    def run(k):
+        rf[_lib.PH_RF_ACTIVE] = 1
+        _lib.call("ph_run3d", plan, _lib.ptr(a), _lib.ptr(b), k, _lib.ptr(rs), _lib.ptr(rf), s.cfl,
+                  _lib.PH_STEP_BANDLIMITED, ws, st)
+
+    run(2)  # warm-up (even: the state is back in a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    run(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nsteps = int(rf[_lib.PH_RF_STEPS].item())
+    dts = float(rs[_lib.PH_RS_DTUSED].item())
+    m = torch.zeros(1, dtype=torch.float64, device=dev)
+    e0.record()
+    for _ in range(steps):
+        _lib.call("ph_max_wave_speed3d", plan, _lib.ptr(a), _lib.ptr(m), None, s.cfl, ws, st)
+    e1.record()
+    torch.cuda.synchronize()
+    sep = e0.elapsed_time(e1) / steps
+    del a, b
+    return {"ms_per_step": round(ms, 3), "steps": nsteps, "last_dt": dts, "cfl_every_step": True,
+            "fused_cfl_max": True, "separate_max_wave_speed_ms": round(sep, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -723,6 +767,9 @@ def main():
     sums = s.conserved_sums(U)
     del out
     extras = {}
+    print("[bench] device run loop (dt from CFL every step)", file=sys.stderr, flush=True)
+    extras["run_loop"] = run_loop_timing(s, U, args.steps)
+    extras["run_loop"]["overhead_vs_fixed_dt_ms"] = round(extras["run_loop"]["ms_per_step"] - ms_per_step, 3)
     if not args.no_extras:
         extras["hbm_reference"] = hbm_reference(dev)
         if roof is not None:

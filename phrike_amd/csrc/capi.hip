@@ -217,8 +217,11 @@ int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, i
         hipLaunchKernelGGL(ph::k_run_clip, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, (const int*)rf_dev);
         ph::check_launch("k_run_clip");
         // steps after the first take a previous step's (band-limited) output
-        p.step(src, dst, 0.0, rs_dev + ph::RS_DT, ws, st, k > 0 || (flags & PH_STEP_BANDLIMITED) != 0, rf_dev);
-        p.max_speed(dst, nullptr, rs_dev + ph::RS_DTNEXT, cfl, ws, st, rf_dev);
+        // the step's last kernel leaves the new state's per-row wave-speed
+        // maxima behind, so the next dt needs no second read of the state
+        p.step(src, dst, 0.0, rs_dev + ph::RS_DT, ws, st, k > 0 || (flags & PH_STEP_BANDLIMITED) != 0, rf_dev,
+               true);
+        p.max_rows(nullptr, rs_dev + ph::RS_DTNEXT, cfl, ws, st, rf_dev);
         hipLaunchKernelGGL(ph::k_run_advance, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, rf_dev);
         ph::check_launch("k_run_advance");
       }
@@ -402,8 +405,8 @@ int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, doubl
         R* dst = static_cast<R*>(k % 2 ? Ua : Ub);
         hipLaunchKernelGGL(ph::k_run_clip, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, (const int*)rf_dev);
         ph::check_launch("k_run_clip");
-        p.step(nbatch, src, dst, 0.0, rs_dev + ph::RS_DT * nbatch, act, true, ws, st);
-        p.max_speed(nbatch, dst, nullptr, rs_dev + ph::RS_DTNEXT * nbatch, cfl, ws, st, act);
+        p.step(nbatch, src, dst, 0.0, rs_dev + ph::RS_DT * nbatch, act, true, ws, st, true);
+        p.max_rows(nbatch, nullptr, rs_dev + ph::RS_DTNEXT * nbatch, cfl, ws, st, act);
         hipLaunchKernelGGL(ph::k_run_advance, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, rf_dev);
         ph::check_launch("k_run_advance");
       }

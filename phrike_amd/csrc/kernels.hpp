@@ -3,7 +3,7 @@
 // One RHS evaluation in 3D is five launches (the 2D form skips the z pass):
 //
 //   k_xfwd   flux (5 conserved -> 15 fluxes, in registers) + real-to-complex
-//            FFT along x of two y-rows at once (z = row0 + i*row1), keeps
+//            FFT along x of one y-row (a half-length complex FFT), keeps
 //            |mx| <= Nx/3, scales by sigma_x/(Nx*Ny*Nz) and multiplies the
 //            x-fluxes by i*kx                         -> A[15][z][y][mx]
 //   k_col    y pass: forward FFT along y of column tiles, keeps |my| <= Ny/3,
@@ -13,7 +13,7 @@
 //            Fz^) on |mz| <= Nz/3, zero elsewhere, inverse FFT along z
 //                                                     -> C[5][z][my][mx] (in B)
 //   k_col    y inverse: zero-filled inverse FFT along y -> D[5][z][y][mx] (in A)
-//   k_xinv   complex-to-real inverse along x of two rows at once, then the RK4
+//   k_xinv   complex-to-real inverse along x of one row, then the RK4
 //            stage combine (or plain output) in registers.
 //
 // Because differentiation, the 2/3 mask and the filter are all diagonal in
@@ -69,18 +69,13 @@ struct Keep {
 // -> physical rows, optionally with the RK4 stage combine) and the fused
 // stage boundary k_xmid.
 //
-// A block owns the two rows y0, y0+1 of one slab; their transforms along x run
-// as one complex transform per field of z = row0 + i*row1.  Thread t owns the
-// x positions t + j*NT (j < XJ) of both rows, so one complex LDS access moves
-// the value of a field at one x in both rows.  The kept-mode count MX
-// (NX/3 + 1 with the 2/3 rule, NX/2 + 1 without) is a compile-time constant;
-// spectral rows move in 16-byte chunks: one FP64 complex, or one pair-planar
-// FP32 pair (re(2m) re(2m+1) im(2m) im(2m+1)).  Only the chunks that hold
-// kept modes are read or written (the column passes skip the row padding).
+// The kept-mode count MX (NX/3 + 1 with the 2/3 rule, NX/2 + 1 without) is a
+// compile-time constant; spectral rows move in 16-byte chunks: one FP64
+// complex, or one pair-planar FP32 pair (re(2m) re(2m+1) im(2m) im(2m+1)).
+// Only the chunks that hold kept modes are read or written (the column passes
+// skip the row padding).
 // ----------------------------------------------------------------------------
 
-// Threads per block of the x kernels: each thread owns 2 (FP64) or 4 (FP32)
-// of the block's 2*NX points, which bounds the per-thread flux and FFT state.
 // radix plan of the x kernels' in-LDS transforms: PlanX (a radix-16 pass for
 // log2 N = 3q+1) or the column kernels' Plan
 #ifndef PH_X_RADIX16
@@ -95,6 +90,7 @@ using XPlan = std::conditional_t<PH_X_RADIX16 != 0, PlanX<N>, Plan<N>>;
 #ifndef PH_XT_MAX_F32
 #define PH_XT_MAX_F32 512
 #endif
+// threads per block of the hybrid solver's x kernel (two rows per block)
 template <typename T, int NX>
 struct XThreads {
   static constexpr int value = sizeof(T) == 8 ? clampi(NX, 64, PH_XT_MAX_F64) : clampi(NX / 2, 64, PH_XT_MAX_F32);
@@ -139,25 +135,6 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #define PH_X_WAVES_PER_EU 1
 #endif
 
-// Single-row x kernels (1, default): a block transforms one row y of every
-// field; the real row is packed as z[p] = x[2p] + i x[2p+1] and transformed
-// with an NX/2-point complex FFT.  0: two rows per block (z = row0 + i row1,
-// NX-point transforms).
-#ifndef PH_XROW
-#define PH_XROW 1
-#endif
-
-template <typename T, int NX, bool DA>
-struct XGeo {
-  static constexpr int NT = XThreads<T, NX>::value;
-  static constexpr int MX = DA ? NX / 3 + 1 : NX / 2 + 1;   // kept x modes
-  static constexpr int K = ColElem<T>::PW;                   // modes per chunk
-  static constexpr int MC = (MX + K - 1) / K;                // chunks holding kept modes
-  static constexpr bool XALL = NX >= NT;                     // every thread owns positions
-  static constexpr int XJ = XALL ? NX / NT : 1;              // x positions per thread
-  static constexpr int ZB = NX - 2 * MX + 1 > 0 ? NX - 2 * MX + 1 : 0;  // zero band of an inverse input
-};
-
 // the 16-byte unit of a spectral row
 struct alignas(16) pair4f {
   float a, b, c, d;
@@ -191,115 +168,6 @@ __device__ __forceinline__ T xquot(T a, T b, T rb) {
   else return a * rb;
 }
 
-#if !PH_XROW
-// Kept half spectra of rows y0, y0+1 of NCI fields, loaded into registers in
-// 16-byte chunks; unsplit() writes z = row0 + i*row1 in natural order to LDS
-// (line l = field l), with zeros in the band MX..NX-MX.
-template <typename T, int NX, bool DA, int NCI>
-struct XSpec {
-  using G = XGeo<T, NX, DA>;
-  using C = typename ChunkT<T, G::K>::type;
-  static constexpr int NTASK = NCI * G::MC;
-  static constexpr int PER = (NTASK + G::NT - 1) / G::NT;
-  C a[PER], b[PER];
-
-  // src: row y0 of field 0; fs and pitch in complex values
-  __device__ __forceinline__ void load(const cx<T>* src, long long fs, int pitch, int nfl, int tid) {
-    const C* s = reinterpret_cast<const C*>(src);
-    const long long fsc = fs / G::K;
-    const int pc = pitch / G::K;
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int t = tid + q * G::NT;
-      a[q] = C{};
-      b[q] = C{};
-      if (NTASK % G::NT == 0 || t < NTASK) {
-        const int l = t / G::MC, m = t - l * G::MC;
-        if (l < nfl) {
-          const C* p = s + l * fsc + m;
-          a[q] = p[0];
-          b[q] = p[pc];
-        }
-      }
-    }
-  }
-
-  template <int LS, int PADB>
-  __device__ __forceinline__ void unsplit(cx<T>* buf, int tid) const {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int t = tid + q * G::NT;
-      if (NTASK % G::NT == 0 || t < NTASK) {
-        const int l = t / G::MC, m = t - l * G::MC;
-#pragma unroll
-        for (int i = 0; i < G::K; ++i) {
-          const int k = m * G::K + i;
-          if (k < G::MX) {
-            const cx<T> A = ch_get(a[q], i), B = ch_get(b[q], i);
-            if (k == 0 || 2 * k == NX) {
-              buf[lds_slot<LS, PADB>(k, l)] = mk(A.x, B.x);
-            } else {
-              buf[lds_slot<LS, PADB>(k, l)] = mk(A.x - B.y, A.y + B.x);
-              buf[lds_slot<LS, PADB>(NX - k, l)] = mk(A.x + B.y, B.x - A.y);
-            }
-          }
-        }
-      }
-    }
-    if constexpr (G::ZB > 0) {
-      constexpr int NZ = NCI * G::ZB;
-      for (int t = tid; t < NZ; t += G::NT) {
-        const int l = t / G::ZB, n = G::MX + (t - l * G::ZB);
-        buf[lds_slot<LS, PADB>(n, l)] = mk(T(0), T(0));
-      }
-    }
-  }
-};
-
-// Split the transforms of z = row0 + i*row1 (LX lines in buf, natural order)
-// into the two rows' half spectra, keep MX modes, scale by sig (and i*k),
-// store lines l < nl to dst (row y0 of the group's first field).
-template <typename T, int NX, bool DA, int LX, int LS, int PADB>
-__device__ __forceinline__ void x_split_store(const cx<T>* buf, cx<T>* dst, long long fs, int pitch, int nl,
-                                              const T* sig, const T* kk, bool ik, int tid) {
-  using G = XGeo<T, NX, DA>;
-  using C = typename ChunkT<T, G::K>::type;
-  constexpr int NTASK = LX * G::MC;
-  C* d = reinterpret_cast<C*>(dst);
-  const long long fsc = fs / G::K;
-  const int pc = pitch / G::K;
-  for (int t = tid; t < NTASK; t += G::NT) {
-    const int l = t / G::MC, m = t - l * G::MC;
-    if (l < nl) {
-      C ca{}, cb{};
-#pragma unroll
-      for (int i = 0; i < G::K; ++i) {
-        const int k = m * G::K + i;
-        if (k < G::MX) {
-          const cx<T> zk = buf[lds_slot<LS, PADB>(k, l)];
-          const cx<T> zn = buf[lds_slot<LS, PADB>((NX - k) & (NX - 1), l)];
-          cx<T> A = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
-          cx<T> B = mk(T(0.5) * (zk.y + zn.y), T(0.5) * (zn.x - zk.x));
-          const T s = sig[k];
-          A = cscale(A, s);
-          B = cscale(B, s);
-          if (ik) {
-            const T kx = kk[k];
-            A = mul_ik(A, kx);
-            B = mul_ik(B, kx);
-          }
-          ch_set(ca, i, A);
-          ch_set(cb, i, B);
-        }
-      }
-      C* p = d + l * fsc + m;
-      p[0] = ca;
-      p[pc] = cb;
-    }
-  }
-}
-
-#endif  // !PH_XROW
 
 // ----------------------------------------------------------------------------
 // x forward
@@ -334,165 +202,6 @@ template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
 
-#if !PH_XROW
-// Fluxes (or copies) of the conserved values q (both rows of the thread's x
-// positions) -> forward x transform of two rows at once -> kept modes to
-// a.out.  Shared by k_xfwd and the fused k_xmid.
-template <typename T, int NX, bool DA, int MODE>
-__device__ __forceinline__ void x_forward(const XFwdArgs<T>& a, int ncomp, const cx<T>* tw, cx<T>* buf,
-                                          const T* sig, const T* kk, int tid, int slab, int y0,
-                                          const T (&q)[XGeo<T, NX, DA>::XJ][2][XFwdShape<MODE>::NC]) {
-  using G = XGeo<T, NX, DA>;
-  using S = XFwdShape<MODE>;
-  constexpr int LX = S::LX, NC = S::NC, NT = G::NT, XJ = G::XJ;
-  // primitive recovery (phrike/equations.py:104-117, 150-164; MHD 922-933)
-  constexpr int NPR = MODE == XF_MHD3 ? 11 : 6;
-  T pr[XJ][2][NPR];
-  if constexpr (MODE == XF_MHD3) {
-    // rho ux uy uz p* Bx By Bz E uB (E + p*)  (phrike/equations.py:922-933, 949-1000)
-#pragma unroll
-    for (int j = 0; j < XJ; ++j)
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const T* u = q[j][r];
-        const T rho = u[0];
-        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
-        const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
-        const T Bx = u[5], By = u[6], Bz = u[7];
-        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-        const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-        const T p = (a.gamma - T(1)) * ((u[4] - kin) - mag);
-        const T ps = p + mag;
-        T* w = pr[j][r];
-        w[0] = rho; w[1] = ux; w[2] = uy; w[3] = uz; w[4] = ps;
-        w[5] = Bx; w[6] = By; w[7] = Bz; w[8] = u[4];
-        w[9] = (ux * Bx + uy * By) + uz * Bz;
-        w[10] = u[4] + ps;
-      }
-  } else if constexpr (MODE != XF_COPY) {
-#pragma unroll
-    for (int j = 0; j < XJ; ++j)
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const T* u = q[j][r];
-        const T rho = u[0];
-        const T ri = sizeof(T) == 8 ? T(0) : xrcp(rho);
-        if constexpr (MODE == XF_FLUX3) {
-          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri), uz = xquot(u[3], rho, ri);
-          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-          const T p = (a.gamma - T(1)) * (u[4] - kin);
-          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = uz;
-          pr[j][r][4] = p;
-          pr[j][r][5] = u[4] + p;  // E + p
-        } else {
-          const T ux = xquot(u[1], rho, ri), uy = xquot(u[2], rho, ri);
-          const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
-          const T p = (a.gamma - T(1)) * (u[3] - kin);
-          pr[j][r][0] = rho; pr[j][r][1] = ux; pr[j][r][2] = uy; pr[j][r][3] = T(0);
-          pr[j][r][4] = p;
-          pr[j][r][5] = u[3] + p;
-        }
-      }
-  }
-  cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y0 * a.mxp;
-  const int nl = MODE == XF_COPY ? ncomp : LX;
-#pragma unroll 1
-  for (int g = 0; g < S::NG; ++g) {
-#pragma unroll
-    for (int j = 0; j < XJ; ++j) {
-      const int x = tid + j * NT;
-      if (G::XALL || x < NX) {
-#pragma unroll
-        for (int l = 0; l < LX; ++l) {
-          T v[2];
-#pragma unroll
-          for (int r = 0; r < 2; ++r) {
-            if constexpr (MODE == XF_COPY) {
-              v[r] = q[j][r][l];
-            } else if constexpr (MODE == XF_MHD3) {
-              // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000)
-              const T* w = pr[j][r];
-              // direction g picked by selects: a register index by the runtime
-              // g would put pr in scratch memory
-              const T ug = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
-              const T Bg = g == 0 ? w[5] : (g == 1 ? w[6] : w[7]);
-              const T mg = w[0] * ug;
-              if (l == 0) v[r] = mg;
-              else if (l <= 3) {
-                v[r] = mg * w[l];
-                if (l == g + 1) v[r] = v[r] + w[4];
-                v[r] = v[r] - Bg * w[4 + l];
-              } else if (l == 4) v[r] = w[10] * ug - Bg * w[9];
-              else v[r] = (l - 5 == g) ? T(0) : ug * w[l] - w[l - 4] * Bg;
-            } else {
-              // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
-              const T* w = pr[j][r];
-              const T ud = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
-              const T md = w[0] * ud;
-              if (l == 0) v[r] = md;
-              else if (l == NC - 1) v[r] = w[5] * ud;
-              else {
-                v[r] = md * w[l];
-                if (l == g + 1) v[r] = v[r] + w[4];
-              }
-            }
-          }
-          buf[xslot<LX>(x, l)] = mk(v[0], v[1]);
-        }
-      }
-    }
-    lds_barrier();
-    fft_lds<T, NX, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<NX>>(buf, tw, tid);
-    x_split_store<T, NX, DA, LX, xstride(LX), PH_XPADB>(buf, dst + (long long)(g * LX) * a.out_fs, a.out_fs,
-                                                        a.mxp, nl, sig, kk, (a.ikmask >> (g * LX)) & 1u, tid);
-    lds_barrier();
-  }
-}
-
-#endif  // !PH_XROW
-
-template <typename T, int NX, bool DA>
-__device__ __forceinline__ void x_tables(const T* sig, const T* kk, T* s_sig, T* s_k, int tid) {
-  using G = XGeo<T, NX, DA>;
-  for (int i = tid; i < G::MX; i += G::NT) {
-    s_sig[i] = sig[i];
-    s_k[i] = kk[i];
-  }
-}
-
-#if !PH_XROW
-template <typename T, int NX, bool DA, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xfwd(XFwdArgs<T> a) {
-  using G = XGeo<T, NX, DA>;
-  using S = XFwdShape<MODE>;
-  constexpr int NT = G::NT, NC = S::NC, XJ = G::XJ;
-  PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xbuf_size<NX, S::LX>()];
-  __shared__ T s_sig[G::MX], s_k[G::MX];
-  const int tid = threadIdx.x;
-  const int slab = blockIdx.y;
-  if (gated_off(a.active, a.act_all, slab)) return;
-  const int y0 = 2 * blockIdx.x;
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  x_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
-  const T* src = a.in + (long long)slab * a.in_ss + (long long)y0 * NX;
-  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
-  T q[XJ][2][NC];
-#pragma unroll
-  for (int j = 0; j < XJ; ++j) {
-    const int x = tid + j * NT;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        q[j][r][c] = T(0);
-        if ((G::XALL || x < NX) && c < ncomp) q[j][r][c] = src[(long long)c * a.in_fs + r * NX + x];
-      }
-  }
-  x_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y0, q);
-}
-
-#endif  // !PH_XROW
 
 // ----------------------------------------------------------------------------
 // column passes (y and z): tiles of TM consecutive mx columns, lines along the
@@ -766,7 +475,73 @@ struct XInvArgs {
   int act_all;             // 1: active[0] gates every slab
   T gamma, rho_min, p_min;
   const cx<T>* tw;
+  // XI_OUT / XI_ADD / XI_STAGE2 stage 4 with the 4 or 5 Euler fields: the
+  // max wave speed of the row just written -> cfl_rows[slab * rows + y]
+  // (null: none)
+  double* cfl_rows;
 };
+
+// Wave speed of one point from its conserved values u[0..DIM+1].  No FMA
+// contraction, so the standalone reduction and the fused one of the step
+// (which takes these values from registers) give the same bits.
+template <typename T, int DIM>
+__device__ __forceinline__ double wave_speed_v(const T* u, T gamma) {
+#pragma clang fp contract(off)
+  // 3D: max(|ux|,|uy|,|uz|) + a (phrike/equations.py:199-211);
+  // 2D: |ux| + a (phrike/equations.py:137-147)
+  const T rho = u[0];
+  if constexpr (DIM == 3) {
+    const T ux = u[1] / rho, uy = u[2] / rho, uz = u[3] / rho;
+    const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+    const T p = (gamma - T(1)) * (u[4] - kin);
+    const T a = sqrt(gamma * p / rho);
+    T m = fabs(ux) > fabs(uy) ? fabs(ux) : fabs(uy);
+    m = m > fabs(uz) ? m : fabs(uz);
+    return double(m + a);
+  } else {
+    const T ux = u[1] / rho, uy = u[2] / rho;
+    const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
+    const T p = (gamma - T(1)) * (u[3] - kin);
+    const T a = sqrt(gamma * p / rho);
+    return double(fabs(ux) + a);
+  }
+}
+template <typename T, int DIM>
+__device__ __forceinline__ double wave_speed(const T* u, long long fs, long long i, T gamma) {
+  T v[DIM + 2];
+#pragma unroll
+  for (int c = 0; c < DIM + 2; ++c) v[c] = u[c * fs + i];
+  return wave_speed_v<T, DIM>(v, gamma);
+}
+
+__device__ __forceinline__ double dmax_nan(double a, double b) {
+  // NaN wins (a NaN speed must not be hidden)
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+// Max of m over the block -> *dst: xor shuffles within each wavefront, then
+// the block's wavefronts through LDS.  Every thread of the block calls it.
+template <int NTB>
+__device__ __forceinline__ void block_max_store(double m, double* dst, int tid) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = dmax_nan(m, __shfl_xor(m, o, 64));
+  constexpr int NW = NTB / 64;
+  if constexpr (NW == 1) {
+    if (tid == 0) *dst = m;
+  } else {
+    __shared__ double wred[NW];
+    if ((tid & 63) == 0) wred[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      double r = wred[0];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) r = dmax_nan(r, wred[w]);
+      *dst = r;
+    }
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ void clamp2d(T* u, T gamma, T rho_min, T p_min) {
@@ -846,98 +621,6 @@ __device__ __forceinline__ void mhd_stage_point(const XInvArgs<T>& a, const T (&
   u[4] = (pc / (a.gamma - T(1)) + (T(0.5) * rc) * ((ux * ux + uy * uy) + uz * uz)) + mag;
 }
 
-// the RK combine's inputs at one point: U (stages 1-3) and the accumulator
-// (stages 2-4)
-template <typename T, int NC>
-__device__ __forceinline__ void rk_inputs(const XInvArgs<T>& a, long long o, T (&un)[NC], T (&acc)[NC]) {
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    un[c] = a.stage < 4 ? a.un[c * a.fs + o] : T(0);
-    acc[c] = a.stage > 1 ? a.out[c * a.fs + o] : T(0);
-  }
-}
-
-#if !PH_XROW
-template <typename T, int NX, bool DA, int NC, int MODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xinv(XInvArgs<T> a) {
-  using G = XGeo<T, NX, DA>;
-  constexpr int NT = G::NT, XJ = G::XJ;
-  PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xbuf_size<NX, NC>()];
-  const int tid = threadIdx.x;
-  const int slab = blockIdx.y;
-  if (gated_off(a.active, a.act_all, slab)) return;
-  const int y0 = 2 * blockIdx.x;
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
-  static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
-  static_assert(MODE != XI_MHDSTAGE || NC == 8, "the MHD stage combine takes the 8 MHD fields");
-  XSpec<T, NX, DA, NC> sp;
-  sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, nfl, tid);
-  sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
-  lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB, XPlan<NX>>(buf, tw, tid);
-  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
-#pragma unroll
-  for (int j = 0; j < XJ; ++j) {
-    const int x = tid + j * NT;
-    if (!G::XALL && x >= NX) continue;
-    T v[2][NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const cx<T> z = buf[xslot<NC>(x, c)];
-      v[0][c] = z.x;
-      v[1][c] = z.y;
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long long o = base + r * NX + x;
-      if constexpr (MODE == XI_OUT) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (c < nfl) a.out[c * a.fs + o] = v[r][c];
-      } else if constexpr (MODE == XI_RHS) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (c < nfl) a.out[c * a.fs + o] = -v[r][c];
-      } else if constexpr (MODE == XI_ADD) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-          if (c < nfl) a.out[c * a.fs + o] = a.out[c * a.fs + o] + v[r][c];
-      } else if constexpr (MODE == XI_PROJE) {
-        // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933
-        T* u = a.out;
-        const T rho = u[o];
-        const T ux = u[a.fs + o] / rho, uy = u[2 * a.fs + o] / rho, uz = u[3 * a.fs + o] / rho;
-        const T b0 = u[5 * a.fs + o], b1 = u[6 * a.fs + o], b2 = u[7 * a.fs + o];
-        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-        const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
-        const T p = (a.gamma - T(1)) * ((u[4 * a.fs + o] - kin) - mag);
-        const T Bx = v[r][0], By = v[r][1], Bz = v[r][2];
-        u[4 * a.fs + o] = (p / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-        u[5 * a.fs + o] = Bx;
-        u[6 * a.fs + o] = By;
-        u[7 * a.fs + o] = Bz;
-      } else if constexpr (MODE == XI_MHDSTAGE) {
-        T u[8];
-        mhd_stage_point<T>(a, v[r], o, dtd, u);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) a.us[c * a.fs + o] = u[c];
-      } else {
-        T un[NC], acc[NC], u[NC];
-        rk_inputs<T, NC>(a, o, un, acc);
-        rk_point<T, NC, MODE == XI_STAGE2>(a, v[r], o, dtd, u, un, acc);
-        T* dst = a.stage < 4 ? a.us : a.out;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) dst[c * a.fs + o] = u[c];
-      }
-    }
-  }
-}
-
-#endif  // !PH_XROW
-
 // Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
 // the fluxes of the stage s+1 state and their forward x transform, or (stage
 // 4) the new state's forward x transform for the post-step filter, which
@@ -953,112 +636,7 @@ struct XMidArgs {
   XFwdArgs<T> fwd_last;  // stage 4 (copy mode, 5 or 4 fields)
 };
 
-#if !PH_XROW
-template <typename T, int NX, bool DA, int NC, int FMODE>
-__global__ __launch_bounds__((XThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xmid(XMidArgs<T> m) {
-  using G = XGeo<T, NX, DA>;
-  constexpr int NT = G::NT, XJ = G::XJ;
-  static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
-  PH_XTW_DECL(T, NX);
-  // inverse (NC lines), forward (FMODE lines) and the stage-4 copy (5 lines)
-  constexpr int LB0 = xstride(NC) > xstride(XFwdShape<FMODE>::LX) ? xstride(NC) : xstride(XFwdShape<FMODE>::LX);
-  constexpr int LB = LB0 > xstride(XFwdShape<XF_COPY>::LX) ? LB0 : xstride(XFwdShape<XF_COPY>::LX);
-  __shared__ cx<T> buf[LdsSize<NX, LB, PH_XPADB>::value];
-  __shared__ T s_sig[G::MX], s_k[G::MX];
-  const XInvArgs<T>& a = m.inv;
-  const int tid = threadIdx.x;
-  const int slab = blockIdx.y;
-  if (gated_off(a.active, a.act_all, slab)) return;
-  const int y0 = 2 * blockIdx.x;
-  XSpec<T, NX, DA, NC> sp;
-  sp.load(a.in + (long long)slab * a.in_ss + (long long)y0 * a.mxp, a.in_fs, a.mxp, NC, tid);
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  // the forward that follows: stages 1-3 fluxes, stage 4 the new state
-  const XFwdArgs<T>& fa = a.stage < 4 ? m.fwd : m.fwd_last;
-  x_tables<T, NX, DA>(fa.sig, fa.kk, s_sig, s_k, tid);
-  const long long base = (long long)slab * a.ss + (long long)y0 * NX;
-  // the RK combine's inputs, loaded before the inverse transform so their
-  // latency overlaps it (PH_XMID_PREFETCH)
-  constexpr int PJ = PH_XMID_PREFETCH ? XJ : 1;
-  T pre_un[PJ][2][NC], pre_acc[PJ][2][NC];
-  if constexpr (PH_XMID_PREFETCH != 0) {
-#pragma unroll
-    for (int j = 0; j < XJ; ++j) {
-      const int x = tid + j * NT;
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          pre_un[j][r][c] = T(0);
-          pre_acc[j][r][c] = T(0);
-          if (G::XALL || x < NX) {
-            const long long o = base + r * NX + x;
-            if (a.stage < 4) pre_un[j][r][c] = a.un[c * a.fs + o];
-            if (PH_XMID_PREFETCH == 1 && a.stage > 1) pre_acc[j][r][c] = a.out[c * a.fs + o];
-          }
-        }
-    }
-  }
-  sp.template unsplit<xstride(NC), PH_XPADB>(buf, tid);
-  lds_barrier();
-  fft_lds<T, NX, NC, +1, NT, xstride(NC), PH_XPADB, XPlan<NX>>(buf, tw, tid);
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
-  T q[XJ][2][5];
-#pragma unroll
-  for (int j = 0; j < XJ; ++j) {
-    const int x = tid + j * NT;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int c = 0; c < 5; ++c) q[j][r][c] = T(0);
-    if (!G::XALL && x >= NX) continue;
-    T v[2][NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const cx<T> z = buf[xslot<NC>(x, c)];
-      v[0][c] = z.x;
-      v[1][c] = z.y;
-    }
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const long long o = base + r * NX + x;
-      T un[NC], acc[NC], u[NC];
-      if constexpr (PH_XMID_PREFETCH != 0) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          un[c] = pre_un[j][r][c];
-          acc[c] = PH_XMID_PREFETCH == 1 ? pre_acc[j][r][c] : (a.stage > 1 ? a.out[c * a.fs + o] : T(0));
-        }
-      } else {
-        rk_inputs<T, NC>(a, o, un, acc);
-      }
-      rk_point<T, NC, NC == 4>(a, v[r], o, dtd, u, un, acc);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) q[j][r][c] = u[c];
-      if (a.stage == 4 && a.keep_state) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) a.out[c * a.fs + o] = u[c];
-      }
-    }
-  }
-  lds_barrier();  // buf is reused by the forward transform
-  if (a.stage < 4) {
-    T qf[XJ][2][XFwdShape<FMODE>::NC];
-#pragma unroll
-    for (int j = 0; j < XJ; ++j)
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
-    x_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y0, qf);
-  } else {
-    x_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y0, q);
-  }
-}
 
-#endif  // !PH_XROW
-
-#if PH_XROW
 // ----------------------------------------------------------------------------
 // Single-row x kernels.  A block owns row y of one slab for every field.  The
 // real row x[0..NX) is packed as z[p] = x[2p] + i x[2p+1] (p < M = NX/2) and
@@ -1464,6 +1042,21 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
     }
   }
   auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+  // the fused CFL reduction of a final state: max wave speed of the values
+  // this thread wrote (u[j][r][c] at x = 2p + r), one partial per row
+  constexpr bool EULER = NC == 4 || NC == 5;
+  auto cfl_store = [&](const T (&u)[PJ][2][NC]) {
+    if constexpr (EULER) {
+      double m = 0.0;
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        if (!G::PALL && G::point(tid, j) >= G::M) continue;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) m = dmax_nan(m, wave_speed_v<T, NC - 2>(u[j][r], a.gamma));
+      }
+      block_max_store<NT>(m, a.cfl_rows + (long long)slab * gridDim.x + y, tid);
+    }
+  };
   if constexpr (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -1475,9 +1068,12 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
         const T s0 = MODE == XI_RHS ? -v[j][0][c] : v[j][0][c];
         const T s1 = MODE == XI_RHS ? -v[j][1][c] : v[j][1][c];
         o[j] = MODE == XI_ADD ? mk(o[j].x + s0, o[j].y + s1) : mk(s0, s1);
+        v[j][0][c] = o[j].x;  // the values written
+        v[j][1][c] = o[j].y;
       }
       xr_store_pairs<G>(row(a.out, c), tid, o);
     }
+    if (MODE != XI_RHS && a.cfl_rows != nullptr) cfl_store(v);
   } else if constexpr (MODE == XI_PROJE || MODE == XI_MHDSTAGE) {
 #pragma unroll
     for (int j = 0; j < PJ; ++j) {
@@ -1546,10 +1142,21 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
         xr_store_pairs<G>(row(a.out, c), tid, us[c]);
       }
     }
+    if (a.stage == 4 && a.cfl_rows != nullptr) {
+      T u[PJ][2][NC];
+#pragma unroll
+      for (int j = 0; j < PJ; ++j)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          u[j][0][c] = us[c][j].x;
+          u[j][1][c] = us[c][j].y;
+        }
+      cfl_store(u);
+    }
   }
 }
 
-// Fused stage boundary (as the two-row form above): x inverse + RK combine
+// Fused stage boundary: x inverse + RK combine
 // of stage s, then the fluxes of the stage s+1 state and their forward x
 // transform (stages 1-3), or the new state's forward x transform for the
 // post-step filter (stage 4).  The block loads its spectral row completely
@@ -1659,16 +1266,11 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
     xr_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
   }
 }
-#endif  // PH_XROW
 
 // launch shape of the x kernels: threads per block and rows per block
 template <typename T, int NX>
 struct XLaunch {
-#if PH_XROW
   static constexpr int threads = XRThreads<T, NX>::value, rows = 1;
-#else
-  static constexpr int threads = XThreads<T, NX>::value, rows = 2;
-#endif
 };
 
 // ----------------------------------------------------------------------------
@@ -1710,35 +1312,6 @@ __global__ __launch_bounds__(NT) void k_spec_in(const T* in, T* B, long long row
 // order (deterministic; no atomics, no memset).
 // ----------------------------------------------------------------------------
 template <typename T, int DIM>
-__device__ __forceinline__ double wave_speed(const T* u, long long fs, long long i, T gamma) {
-  // 3D: max(|ux|,|uy|,|uz|) + a (phrike/equations.py:199-211);
-  // 2D: |ux| + a (phrike/equations.py:137-147)
-  const T rho = u[i];
-  if constexpr (DIM == 3) {
-    const T ux = u[fs + i] / rho, uy = u[2 * fs + i] / rho, uz = u[3 * fs + i] / rho;
-    const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-    const T p = (gamma - T(1)) * (u[4 * fs + i] - kin);
-    const T a = sqrt(gamma * p / rho);
-    T m = fabs(ux) > fabs(uy) ? fabs(ux) : fabs(uy);
-    m = m > fabs(uz) ? m : fabs(uz);
-    return double(m + a);
-  } else {
-    const T ux = u[fs + i] / rho, uy = u[2 * fs + i] / rho;
-    const T kin = (T(0.5) * rho) * (ux * ux + uy * uy);
-    const T p = (gamma - T(1)) * (u[3 * fs + i] - kin);
-    const T a = sqrt(gamma * p / rho);
-    return double(fabs(ux) + a);
-  }
-}
-
-__device__ __forceinline__ double dmax_nan(double a, double b) {
-  // NaN wins (a NaN speed must not be hidden)
-  if (a != a) return a;
-  if (b != b) return b;
-  return a > b ? a : b;
-}
-
-template <typename T, int DIM>
 __global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long fs, long long ss,
                                                          long long n, T gamma, double* part,
                                                          const int* active) {
@@ -1749,6 +1322,23 @@ __global__ __launch_bounds__(NT) void k_maxspeed_partial(const T* u, long long f
   const bool on = active == nullptr || active[blockIdx.y] != 0;
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; on && i < n; i += (long long)gridDim.x * NT)
     m = dmax_nan(m, wave_speed<T, DIM>(us, fs, i, gamma));
+  red[threadIdx.x] = m;
+  lds_barrier();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = dmax_nan(red[threadIdx.x], red[threadIdx.x + s]);
+    lds_barrier();
+  }
+  if (threadIdx.x == 0) part[(long long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// first level of a large partial reduction: n values (per slab, slab stride
+// n) -> gridDim.x maxima per slab, part[slab * gridDim.x + blockIdx.x]
+static __global__ __launch_bounds__(NT) void k_max_mid(const double* in, long long n, double* part) {
+  __shared__ double red[NT];
+  const double* ps = in + (long long)blockIdx.y * n;
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT)
+    m = dmax_nan(m, ps[i]);
   red[threadIdx.x] = m;
   lds_barrier();
   for (int s = NT / 2; s > 0; s >>= 1) {

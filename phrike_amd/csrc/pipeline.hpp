@@ -287,7 +287,7 @@ struct Plan3D : PlanBase {
   T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
   bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
   // workspace layout (bytes)
-  size_t offA, offB, offUs, offPart, wsBytes;
+  size_t offA, offB, offUs, offPart, offCfl, wsBytes;
   static constexpr int NPART = 1024;
 
   // na / nb: fields of the spectral buffers A (x pass output) and B (y pass
@@ -337,7 +337,8 @@ struct Plan3D : PlanBase {
     offB = al(offA + szA);
     offUs = offB;  // unused: the fused step keeps stage states in registers
     offPart = al(offB + szB);
-    wsBytes = al(offPart + sizeof(double) * NPART * MON_NRED);
+    offCfl = al(offPart + sizeof(double) * NPART * MON_NRED);
+    wsBytes = al(offCfl + sizeof(double) * (size_t)ny * nz);
   }
   ~Plan3D() override {
     if (xcol_sm1) (void)hipFree(xcol_sm1);
@@ -351,6 +352,8 @@ struct Plan3D : PlanBase {
   cx<T>* B(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offB); }
   T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
   double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
+  // per-row wave-speed maxima of the state the last step wrote (ny*nz)
+  double* CflRows(void* ws) const { return reinterpret_cast<double*>((char*)ws + offCfl); }
 
   long long a_fs() const { return (long long)nz * ny * mxp; }
   long long b_fs() const { return (long long)nz * ay.nr * mxp; }
@@ -592,8 +595,11 @@ struct Plan3D : PlanBase {
   // `gate` (device int, or null): when gate[0] == 0 every kernel of the step
   // returns at once and out is left as it is (the device-resident run loop
   // keeps launching steps after its run has stopped)
+  // cfl: also leave the new state's per-row wave-speed maxima in CflRows(ws)
+  // (the last kernel takes them from the values it writes); max_rows() then
+  // gives the max and dt without reading the state again
   void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st,
-            bool bandlimited = false, const int* gate = nullptr) const {
+            bool bandlimited = false, const int* gate = nullptr, bool cfl = false) const {
     const bool corr = bandlimited && sizeof(T) == 4;
     xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st, gate);
     for (int s = 1; s <= 4; ++s) {
@@ -620,6 +626,7 @@ struct Plan3D : PlanBase {
     XInvArgs<T> a = xinv_args(ws, gate);
     a.nf = 5;
     a.out = out;
+    a.cfl_rows = cfl ? CflRows(ws) : nullptr;
     if (corr) {
       yfwd(false, 5, 0, ws, st, true, gate);
       zcorr(5, ws, st, gate);
@@ -680,6 +687,18 @@ struct Plan3D : PlanBase {
                        gate);
     check_launch("k_max_final");
   }
+  // max and dt of the state written by the last step(..., cfl = true)
+  void max_rows(double* out, double* dt_out, double cfl, void* ws, hipStream_t st, const int* gate = nullptr) const {
+    double* part = Part(ws);
+    const long long nrow = (long long)ny * nz;
+    const int nb = (int)std::min<long long>(NPART, (nrow + NT - 1) / NT);
+    hipLaunchKernelGGL(k_max_mid, dim3(nb, 1), dim3(NT), 0, st, (const double*)CflRows(ws), nrow, part);
+    check_launch("k_max_mid");
+    const double h = std::min(lx / nx, std::min(ly / ny, lz / nz));
+    hipLaunchKernelGGL((k_max_final<3>), dim3(1), dim3(NT), 0, st, (const double*)part, nb, out, dt_out, cfl * h,
+                       gate);
+    check_launch("k_max_final(rows)");
+  }
   void sums(const T* U, double* out, void* ws, hipStream_t st) const {
     double* part = Part(ws);
     const int nb = (int)std::min<long long>(NPART, (ncell + NT - 1) / NT);
@@ -716,7 +735,7 @@ struct Plan2D : PlanBase {
   DevAxis<CE> dy;
   int mx, mxp;
   long long ncell;
-  size_t offA, offUs, offPart, wsBytes;
+  size_t offA, offUs, offPart, offCfl, wsBytes;
   static constexpr int NPART = 256;
 
   Plan2D(int nx_, int ny_, double lx_, double ly_, bool dealias, bool filt, int p, double alpha,
@@ -739,7 +758,8 @@ struct Plan2D : PlanBase {
     offA = 0;
     offUs = al(8ull * batch * ny * mxp * cs);
     offPart = al(offUs + 4ull * batch * ncell * sizeof(T));
-    wsBytes = al(offPart + sizeof(double) * (size_t)NPART * MON_NRED * batch);
+    offCfl = al(offPart + sizeof(double) * (size_t)NPART * MON_NRED * batch);
+    wsBytes = al(offCfl + sizeof(double) * (size_t)batch * ny);
   }
   ~Plan2D() override {
     dx.release();
@@ -748,6 +768,8 @@ struct Plan2D : PlanBase {
   cx<T>* A(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offA); }
   T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
   double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
+  // per-row wave-speed maxima of the states the last step wrote ([run][y])
+  double* CflRows(void* ws) const { return reinterpret_cast<double*>((char*)ws + offCfl); }
 
   void check_nb(int nb) const {
     if (nb < 1 || nb > batch) throw Error(-2, "nbatch must be in [1, plan batch]");
@@ -853,8 +875,10 @@ struct Plan2D : PlanBase {
   // RK4 with clamp (phrike/solver.py:502-513) and the filter when asked
   // (cadence decided by the caller, phrike/solver.py:645-647); stage
   // boundaries fused as in the 3D step
+  // cfl: also leave the per-row wave-speed maxima of the new states in
+  // CflRows(ws) for max_rows()
   void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active,
-            bool filter, void* ws, hipStream_t st) const {
+            bool filter, void* ws, hipStream_t st, bool cfl = false) const {
     xfwd(XF_FLUX2, U, nb, 0xFu, ws, st, active);
     for (int s = 1; s <= 4; ++s) {
       ypass(true, 0, nb, ws, st, active);
@@ -869,6 +893,7 @@ struct Plan2D : PlanBase {
       m.inv.dt_per_slab = 1;
       m.inv.active = active;
       if (s == 4 && !filter) {
+        m.inv.cfl_rows = cfl ? CflRows(ws) : nullptr;
         launch_xinv<T, 4, XI_STAGE2>(nx, ny, nb, m.inv, st);
         check_launch("k_xinv(2d stage 4)");
         return;
@@ -883,8 +908,17 @@ struct Plan2D : PlanBase {
     a.nf = 4;
     a.out = out;
     a.active = active;
+    a.cfl_rows = cfl ? CflRows(ws) : nullptr;
     launch_xinv<T, 4, XI_OUT>(nx, ny, nb, a, st);
     check_launch("k_xinv(2d filter)");
+  }
+  // max and dt per run of the states written by the last step(..., cfl = true)
+  void max_rows(int nb, double* out, double* dt_out, double cfl, void* ws, hipStream_t st,
+                const int* active = nullptr) const {
+    const double h = std::min(lx / nx, ly / ny);
+    hipLaunchKernelGGL((k_max_final<2>), dim3(nb), dim3(NT), 0, st, (const double*)CflRows(ws), ny, out, dt_out,
+                       cfl * h, active);
+    check_launch("k_max_final(2d rows)");
   }
   void max_speed(int nb, const T* U, double* out, double* dt_out, double cfl, void* ws,
                  hipStream_t st, const int* active = nullptr) const {

@@ -93,6 +93,21 @@ void launch(K kernel, dim3 grid, dim3 block, A... args) {
 #define blockDim (::emu::bDim)
 #define gridDim (::emu::gDim)
 inline void __syncthreads() { ::emu::sync(); }
+// a lane exchange within a wavefront of `width` lanes: every thread of the
+// block must call it (as at a barrier); each deposits its value, then reads
+// its partner's
+namespace emu {
+inline std::vector<double> lane_slots;
+}
+inline double __shfl_xor(double v, int mask, int width = 64) {
+  const int t = ::emu::cur;
+  if (::emu::lane_slots.size() < ::emu::fibers.size()) ::emu::lane_slots.resize(::emu::fibers.size());
+  ::emu::lane_slots[t] = v;
+  ::emu::sync();
+  const double r = ::emu::lane_slots[(t & ~(width - 1)) | ((t & (width - 1)) ^ mask)];
+  ::emu::sync();
+  return r;
+}
 #define __builtin_amdgcn_s_barrier() __syncthreads()
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))

@@ -277,3 +277,40 @@ def test_fft3d_half_spectrum(emu, dtype, tol):
     # a plan with the 2/3 rule refuses
     d = Emu3D(emu, nx, ny, nz, dtype=dtype)
     assert emu.ph_fft3d(d.h, 2, P(f), P(spec), P(d.ws), None) != 0
+
+
+def test_fused_cfl_equals_standalone_max(emu):
+    """The run loops take the next dt from wave-speed maxima the step's last
+    kernel leaves per row (no second read of the state); that dt equals the
+    standalone ph_max_wave_speed* of the written state bit for bit, in 3D
+    (FP64 and FP32) and for batched 2D runs (an inactive run keeps its dt)."""
+    from emu_lib import Emu2D, Emu3D, P
+    from phrike_amd.initial_conditions import khi_2d_periodic, taylor_green_3d
+    cfl = 0.4
+    for dt_ in (np.float64, np.float32):
+        e = Emu3D(emu, 16, 8, 32, dtype=dt_)
+        a = np.ascontiguousarray(taylor_green_3d(16, 8, 32).astype(dt_))
+        b = np.zeros_like(a)
+        rs = np.zeros(6)
+        rf = np.array([1, 0], dtype=np.int32)
+        rs[2], rs[3] = 1.0, np.inf
+        assert emu.ph_max_wave_speed3d(e.h, P(a), None, P(rs[1:]), cfl, P(e.ws), None) == 0
+        assert emu.ph_run3d(e.h, P(a), P(b), 1, P(rs), P(rf), cfl, 0, P(e.ws), None) == 0
+        d = np.zeros(1)
+        assert emu.ph_max_wave_speed3d(e.h, P(b), None, P(d), cfl, P(e.ws), None) == 0
+        assert rf[1] == 1 and rs[1] == d[0] and rs[4] == d[0], (dt_, rs, d)
+    nb = 3
+    e = Emu2D(emu, 32, 16, batch=nb, dtype=np.float32)
+    U = np.ascontiguousarray(np.stack([khi_2d_periodic(32, 16, seed=s) for s in range(nb)]).astype(np.float32))
+    V = np.zeros_like(U)
+    rs = np.zeros(6 * nb)
+    rf = np.zeros(2 * nb, dtype=np.int32)
+    rf[:nb] = [1, 0, 1]
+    rs[2 * nb:3 * nb] = 1.0
+    rs[3 * nb:4 * nb] = np.inf
+    _, d0 = e.dt(U, cfl)
+    rs[nb:2 * nb] = d0
+    rs[nb + 1] = 0.125  # the inactive run's dt stays
+    assert emu.ph_run2d(e.h, nb, P(U), P(V), 1, P(rs), P(rf), cfl, P(e.ws), None) == 0
+    _, d1 = e.dt(V, cfl)
+    assert rs[nb] == d1[0] and rs[nb + 2] == d1[2] and rs[nb + 1] == 0.125
