@@ -468,7 +468,8 @@ def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0, 
     sizes = {128: 0.4, 256: 0.3, 512: 0.2, 1024: 0.1}
     budget = int(budget_gb * 2 ** 30)
 
-    def run(reqs, realtime):
+    def run(reqs, realtime, label):
+        print(f"[bench] ensemble {label}: {len(reqs)} requests", file=sys.stderr, flush=True)
         srv = EnsembleServer(hbm_budget=budget, precision="single", steps_per_replay=32, sizes=tuple(sizes))
         t0 = time.perf_counter()
         res = srv.serve(reqs, realtime=realtime)
@@ -481,19 +482,19 @@ def ensemble_study(rate: float, window: float, budget_gb: float, seed: int = 0, 
     out = {"workload": "KHI 2D sizes 128..1024 (0.4/0.3/0.2/0.1), t_end U[0.5,2], FP32, HBM pool "
                        f"{budget_gb} GB", "hbm_budget_gb": budget_gb}
     reqs = poisson_requests(rate, window, sizes, seed=seed)
-    res, wall, s = run(reqs, True)
+    res, wall, s = run(reqs, True, "stated stream")
     out["stated"] = {"offered_req_per_s": rate, "window_s": window, "requests": len(reqs),
                      "completed_req_per_s": round(len(res) / wall, 3), **s}
     # closed loop: service rate of the size mix
     sat = poisson_requests(1000.0, 0.06, sizes, seed=seed + 1)
-    res, wall, s = run(sat, False)
+    res, wall, s = run(sat, False, "saturation")
     mu = len(res) / wall
     out["saturation"] = {"requests": len(sat), "service_req_per_s": round(mu, 3), **s}
     lam = round(0.7 * mu, 3)
     sus = {"offered_req_per_s": lam}
     for w in sustain_windows:
         reqs = poisson_requests(lam, w, sizes, seed=seed + 2)
-        res, wall, s = run(reqs, True)
+        res, wall, s = run(reqs, True, f"{lam} req/s over {w} s")
         sus[f"window_{int(w)}s"] = {"requests": len(reqs), "completed_req_per_s": round(len(res) / wall, 3), **s}
     p99s = [sus[f"window_{int(w)}s"]["p99_s"] for w in sustain_windows]
     sus["p99_growth"] = round(p99s[-1] / p99s[0], 3) if p99s[0] else None
@@ -644,6 +645,8 @@ def main():
     ap.add_argument("--ensemble-window", type=float, default=3.0)
     ap.add_argument("--ensemble-rate", type=float, default=20.0)
     ap.add_argument("--ensemble-budget-gb", type=float, default=8.0)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the ensemble serving study (stated stream, saturation, sustainable rate)")
     ap.add_argument("--gll-only", action="store_true",
                     help="only run the batched D.U matrix-core study (for profiling the MFMA kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -687,6 +690,10 @@ def main():
         return
     if args.gll_only:
         print(json.dumps({"gll_batched": gll_saturation(torch.device("cuda:0"))}))
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_study(args.ensemble_rate, args.ensemble_window,
+                                                     args.ensemble_budget_gb)}))
         return
     from phrike_amd import _lib
     from phrike_amd.equations import EulerEquations3D

@@ -21,9 +21,10 @@ ph_max_wave_speed2d on the admitted slots alone).
 Admission by memory: with ``hbm_budget`` the buckets draw their slots from one
 pool of HBM.  Between replays a bucket whose queue exceeds its free slots
 grows (its buffers are reallocated with the active runs copied over and its
-graph captured again) as far as the pool allows, and a bucket that is mostly
-idle gives slots back, so memory left idle by small runs admits a queued
-large run.  With ``capacities`` the slot counts are fixed instead.
+graph captured again) as far as the pool allows; when the free part of the
+pool does not suffice, mostly idle buckets give slots back, so memory left
+idle by small runs admits a queued large run.  With ``capacities`` the slot
+counts are fixed instead.
 
 A run's arithmetic does not depend on the other slots (every kernel and
 reduction is per run), so a result is bit-identical alone and in a mix, in
@@ -268,13 +269,14 @@ class EnsembleServer:
     def __init__(self, capacities: Optional[Dict[int, int]] = None, precision: str = "single", cfl: float = 0.4,
                  steps_per_replay: int = 16, filter_params: Optional[Dict] = None, device: str = "cuda:0",
                  use_graph: bool = True, keep_states: bool = False, hbm_budget: Optional[int] = None,
-                 sizes: Sequence[int] = (128, 256, 512, 1024), max_slots: int = 64):
+                 sizes: Sequence[int] = (128, 256, 512, 1024), max_slots: int = 64, min_grow: int = 4):
         self.dev = torch.device(device)
         self.precision = precision
         self.use_graph = use_graph
         self.keep_states = keep_states
         self.budget = hbm_budget
         self.max_slots = int(max_slots)
+        self.min_grow = max(1, int(min_grow))
         if capacities is None and hbm_budget is None:
             raise ValueError("give capacities or hbm_budget")
         if capacities is not None:
@@ -290,20 +292,33 @@ class EnsembleServer:
     def hbm_bytes(self) -> int:
         return sum(slot_bytes(b.n, self.precision) * b.cap for b in self.buckets.values())
 
+    def _reclaim(self, need: int, skip: int, queues) -> int:
+        """Shrink idle buckets (no queue, no replay in flight, at most a
+        quarter occupied) until `need` bytes are free; returns the bytes
+        freed.  Memory is only taken back under pressure, so buckets do not
+        shrink and regrow (each resize captures the bucket's graph again)."""
+        freed = 0
+        for n, b in sorted(self.buckets.items(), key=lambda kv: -slot_bytes(kv[0], self.precision) * kv[1].cap):
+            if freed >= need:
+                break
+            if n == skip or b.inflight or queues[n] or b.cap == 0:
+                continue
+            occ = b.n_active()
+            new = 0 if occ == 0 else _pow2_ceil(occ)
+            if new < b.cap and occ <= b.cap // 4:
+                freed += (b.cap - new) * slot_bytes(n, self.precision)
+                b.resize(new)
+        return freed
+
     def _rebalance(self, queues) -> bool:
-        """Pool mode: grow buckets with queued runs as far as the budget
-        allows (powers of two up to max_slots), shrink idle ones.  Only
-        buckets without a replay in flight are touched."""
+        """Pool mode: a bucket whose queue exceeds its free slots grows
+        (powers of two, at least ``min_grow`` slots, at most max_slots) as
+        far as the budget allows, taking memory back from idle buckets only
+        when the free part of the budget does not suffice.  Only buckets
+        without a replay in flight are resized."""
         if self.fixed:
             return False
         changed = False
-        for n, b in self.buckets.items():
-            if b.inflight or queues[n]:
-                continue
-            occ = b.n_active()
-            if b.cap > 0 and occ <= b.cap // 4:
-                b.resize(0 if occ == 0 else _pow2_ceil(occ))
-                changed = True
         for n, b in sorted(self.buckets.items(), key=lambda kv: -len(queues[kv[0]])):
             if b.inflight or not queues[n]:
                 continue
@@ -311,14 +326,18 @@ class EnsembleServer:
             need = occ + len(queues[n])
             if need <= b.cap:
                 continue
-            target = min(self.max_slots, _pow2_ceil(need))
             per = slot_bytes(n, self.precision)
-            free = self.budget - (self.hbm_bytes() - per * b.cap)
-            while target > b.cap and target * per > free:
+            target = min(self.max_slots, max(self.min_grow, _pow2_ceil(need)))
+            if target <= b.cap:
+                continue
+            free = self.budget - self.hbm_bytes()
+            if (target - b.cap) * per > free:
+                free += self._reclaim((target - b.cap) * per - free, n, queues)
+            while target > b.cap and (target - b.cap) * per > free:
                 target //= 2
             if target <= b.cap:
-                # no room for a power of two: any growth that fits
-                target = min(need, self.max_slots, int(free // per))
+                # no power of two fits: any growth that does
+                target = min(need, self.max_slots, b.cap + int(free // per))
             if target > b.cap:
                 b.resize(target)
                 changed = True
