@@ -174,17 +174,15 @@ def precision_study(n: int, steps: int, dev):
             "energy_drift_f32": float(f"{(f1[4] - f0[4]) / f0[4]:.3e}")}
 
 
-def torch_backend_reference(n: int, steps: int, dev, dt: float, hip_state=None):
+def torch_backend_step(n: int, dev):
     """The reference's own GPU path (its `torch` backend: phrike/grid.py:1424-1518,
     1705-1726; phrike/equations.py:582-607; phrike/solver.py:1177-1234), restated
-    with torch.fft (the vendor FFT library) on this GPU, FP64, TGV n^3: full
-    complex fftn of all 5 fields per derivative, full 3D mask and sigma
-    tensors, ifftn(...).real, RK4 with state-sized temporaries, post-step
-    filter.  A second GPU reference point, timed over `steps` steps after one
-    warm-up step; with `hip_state` (this backend's result after the same steps
-    from the same start) the relative L2 difference is reported too."""
+    with torch.fft (the vendor FFT library), FP64, on [0, 2pi)^3 with n^3
+    points, 2/3 mask and the exponential filter (alpha 36, p 16): full complex
+    fftn of all 5 fields per derivative, full 3D mask and sigma tensors,
+    ifftn(...).real, RK4 with state-sized temporaries, post-step filter.
+    Returns step(U, dt)."""
     import torch
-    from phrike_amd.initial_conditions import taylor_green_3d_torch
 
     gam = 1.4
     L = 2 * np.pi
@@ -227,6 +225,18 @@ def torch_backend_reference(n: int, steps: int, dev, dt: float, hip_state=None):
         Un = U + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
         return torch.fft.ifftn(masked(torch.fft.fftn(Un, dim=dims)), dim=dims).real
 
+    return step
+
+
+def torch_backend_reference(n: int, steps: int, dev, dt: float, hip_state=None):
+    """torch_backend_step on TGV n^3 as a second GPU reference point, timed
+    over `steps` steps after one warm-up step; with `hip_state` (this
+    backend's result after the same steps from the same start) the relative
+    L2 difference is reported too."""
+    import torch
+    from phrike_amd.initial_conditions import taylor_green_3d_torch
+
+    step = torch_backend_step(n, dev)
     U0 = taylor_green_3d_torch(n, dev, torch.float64)
     U = step(U0, dt)  # warm-up (FFT plans)
     torch.cuda.synchronize()
@@ -245,17 +255,24 @@ def torch_backend_reference(n: int, steps: int, dev, dt: float, hip_state=None):
     return out
 
 
-def hybrid_study(nx: int, nys, steps: int, dev):
+def hybrid_study(nx: int, nys, steps: int, dev, ensembles=(16, 64), check=200):
     """2D Kelvin-Helmholtz on the hybrid grid (Fourier x on [0,1), LGL y on
     [-1/2, 1/2]): shear layers at y = +-1/4 of width 0.05, rho 1 -> 2, p = 2.5,
-    v = 0.01 sin(4 pi x) + 1e-4 N(0,1) (seed 7), walls at y = +-1/2; `steps`
-    RK4 steps with dt frozen from CFL 0.3 at t = 0 (CFL 0.4 with this filter
+    v = 0.01 sin(4 pi x) + 1e-4 N(0,1) (seed 7), walls at y = +-1/2; RK4
+    steps with dt frozen from CFL 0.3 at t = 0 (CFL 0.4 with this filter
     turns unstable at the walls after ~850 steps at 256 x 128), post-step
-    x/y filters.  FP64 with the FP64
-    matrix-core D, then the same with split-precision D (FP32 matrix cores on
-    hi/lo parts of D and of the FP64 data) and with an FP32 state; reports
-    per-field relative L2 against FP64, time per step, and the D.U kernel
-    rate (useful flops 2 Ny^2 * 4 Nx per apply)."""
+    x/y filters.  Three arithmetic forms of D.Fy: FP64 matrix cores on FP64
+    state ("f64"), split precision (FP32 matrix cores on hi/lo parts of D
+    and of the data) on FP64 state ("split_f64state") and on FP32 state
+    ("split_f32state").
+
+    * one run per size: time per step, D.U kernel rate (useful flops
+      2 Ny^2 * 4 Nx per apply), relative L2 per field against FP64 after
+      `check` and after `steps` steps, next to an FP64 run whose initial state
+      is perturbed by 1e-15 relative (the KHI's own growth of a rounding-size
+      difference);
+    * ensembles of runs (batched, the D.U apply then takes the tiled
+      matrix-core kernel): time per step of the batch and per run-step."""
     import torch
     from phrike_amd import _lib
     from phrike_amd.equations import EulerEquations2D
@@ -263,53 +280,87 @@ def hybrid_study(nx: int, nys, steps: int, dev):
     from phrike_amd.initial_conditions import khi_2d_gll
 
     fp = {"enabled": True, "p": 16, "alpha": 36.0}
+    forms = (("f64", "double", "f64"), ("split_f64state", "double", "split"), ("split_f32state", "single", "split"))
     out = {}
+
+    def kernel_rate(sol, U0, dt, ny, nb=1):
+        _lib.timing_collect()
+        _lib.timing_enable(True)
+        sol.advance(U0, 20, dt=dt)
+        torch.cuda.synchronize()
+        _lib.timing_enable(False)
+        kt = _lib.timing_collect()
+        if "h_gll_d" not in kt:
+            return None
+        cnt, tms = kt["h_gll_d"]
+        flops = 2.0 * ny * ny * 4 * nx * nb
+        return {"avg_us": round(tms / cnt * 1e3, 2), "useful_TFLOPs": round(flops / (tms / cnt / 1e3) / 1e12, 3)}
+
     for ny in nys:
-        finals, ms, dk = {}, {}, {}
+        snaps, ms, dk = {}, {}, {}
         dt = None
-        for name, prec, gll in (("f64", "double", "f64"), ("split_f64state", "double", "split"),
-                                ("split_f32state", "single", "split")):
+        runs = list(forms) + [("f64_perturbed", "double", "f64")]
+        for name, prec, gll in runs:
             g = HybridGrid2D(nx, ny, filter_params=fp, precision=prec, gll=gll, torch_device=str(dev))
             sol = HybridSolver2D(g, EulerEquations2D(1.4), cfl=0.3)
-            U0 = torch.as_tensor(khi_2d_gll(nx, g.y_nodes, seed=7), device=dev).to(g.dtype)
+            U0 = torch.as_tensor(khi_2d_gll(nx, g.y_nodes, seed=7), device=dev)
+            if name == "f64_perturbed":
+                gen = torch.Generator(device="cpu").manual_seed(11)
+                U0 = U0 * (1.0 + 1e-15 * torch.randn(U0.shape, generator=gen, dtype=torch.float64).to(dev))
+            U0 = U0.to(g.dtype)
             if dt is None:
                 dt = sol.compute_dt(U0)
             sol.advance(U0, 2, dt=dt)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            U = sol.advance(U0, steps, dt=dt)
+            U1 = sol.advance(U0, check, dt=dt)
+            U2 = sol.advance(U1, steps - check, dt=dt)
             torch.cuda.synchronize()
             ms[name] = (time.perf_counter() - t0) / steps * 1e3
-            # per-kernel times from a short separate pass (events per launch)
-            _lib.timing_collect()
-            _lib.timing_enable(True)
-            sol.advance(U0, 20, dt=dt)
-            torch.cuda.synchronize()
-            _lib.timing_enable(False)
-            kt = _lib.timing_collect()
-            if "h_gll_d" in kt:
-                cnt, tms = kt["h_gll_d"]
-                flops = 2.0 * ny * ny * 4 * nx
-                dk[name] = {"avg_us": round(tms / cnt * 1e3, 2),
-                            "useful_TFLOPs": round(flops / (tms / cnt / 1e3) / 1e12, 3)}
-            finals[name] = U.double()
+            if name != "f64_perturbed":
+                dk[name] = kernel_rate(sol, U0, dt, ny)
+            snaps[name] = (U1.double(), U2.double())
             del g, sol
-        ref = finals["f64"]
+        ref = snaps["f64"]
 
-        def errs(a):
-            return [float(f"{float(torch.linalg.vector_norm(a[f] - ref[f]) / torch.linalg.vector_norm(ref[f])):.3e}")
+        def errs(k, i):
+            a, r = snaps[k][i], ref[i]
+            return [float(f"{float(torch.linalg.vector_norm(a[f] - r[f]) / torch.linalg.vector_norm(r[f])):.3e}")
                     for f in range(4)]
 
-        mom = max(float(torch.linalg.vector_norm(ref[1])), float(torch.linalg.vector_norm(ref[2])))
-        out[f"{nx}x{ny}"] = {"ms_per_step": {k: round(v, 4) for k, v in ms.items()},
-                             "finite": bool(torch.isfinite(ref).all()),
-                             "rel_l2_vs_f64": {k: errs(finals[k]) for k in ("split_f64state", "split_f32state")},
-                             "my_err_on_momentum_scale": {
-                                 k: float(f"{float(torch.linalg.vector_norm(finals[k][2] - ref[2])) / mom:.3e}")
-                                 for k in ("split_f64state", "split_f32state")},
-                             "gll_d_kernel": dk}
+        others = ("split_f64state", "split_f32state", "f64_perturbed")
+        out[f"{nx}x{ny}"] = {
+            "ms_per_step": {k: round(v, 4) for k, v in ms.items()},
+            "finite": bool(torch.isfinite(ref[1]).all()),
+            f"rel_l2_vs_f64_at_{check}": {k: errs(k, 0) for k in others},
+            f"rel_l2_vs_f64_at_{steps}": {k: errs(k, 1) for k in others},
+            "gll_d_kernel": dk}
+    # ensembles of the first size
+    ny = nys[0]
+    ens = {}
+    for nb in ensembles:
+        row = {}
+        for name, prec, gll in forms:
+            g = HybridGrid2D(nx, ny, filter_params=fp, precision=prec, gll=gll, batch=nb, torch_device=str(dev))
+            sol = HybridSolver2D(g, EulerEquations2D(1.4), cfl=0.3)
+            U0 = torch.stack([torch.as_tensor(khi_2d_gll(nx, g.y_nodes, seed=7 + i), device=dev)
+                              for i in range(nb)]).to(g.dtype)
+            dt = sol.compute_dt(U0[0])
+            sol.advance(U0, 2, dt=dt)
+            torch.cuda.synchronize()
+            k = 50
+            t0 = time.perf_counter()
+            sol.advance(U0, k, dt=dt)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / k * 1e3
+            row[name] = {"ms_per_step": round(t, 4), "us_per_run_step": round(t * 1e3 / nb, 2),
+                         "gll_d_kernel": kernel_rate(sol, U0, dt, ny, nb)}
+            del g, sol, U0
+            torch.cuda.empty_cache()
+        ens[str(nb)] = row
+    out[f"ensemble_{nx}x{ny}"] = ens
     return {"workload": f"KHI 2D hybrid Fourier-x ({nx}) x LGL-y with walls, {steps} RK4 steps, "
-                        f"dt frozen, CFL 0.3",
+                        f"dt frozen, CFL 0.3; ensembles of {list(ensembles)} runs",
             **out}
 
 
@@ -645,6 +696,9 @@ def main():
     ap.add_argument("--ensemble-window", type=float, default=3.0)
     ap.add_argument("--ensemble-rate", type=float, default=20.0)
     ap.add_argument("--ensemble-budget-gb", type=float, default=8.0)
+    ap.add_argument("--hybrid-only", action="store_true",
+                    help="only run the hybrid KHI study (split-precision D.U, ensembles)")
+    ap.add_argument("--mhd-only", action="store_true", help="only run the ideal-MHD study")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the ensemble serving study (stated stream, saturation, sustainable rate)")
     ap.add_argument("--gll-only", action="store_true",
@@ -690,6 +744,12 @@ def main():
         return
     if args.gll_only:
         print(json.dumps({"gll_batched": gll_saturation(torch.device("cuda:0"))}))
+        return
+    if args.hybrid_only:
+        print(json.dumps({"hybrid_khi": hybrid_study(256, (128, 256), args.hybrid_steps, torch.device("cuda:0"))}))
+        return
+    if args.mhd_only:
+        print(json.dumps({"mhd3d": mhd_study(args.mhd_size, args.mhd_steps, torch.device("cuda:0"))}))
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_study(args.ensemble_rate, args.ensemble_window,
