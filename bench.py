@@ -369,8 +369,11 @@ def mhd_study(n: int, steps: int, dev):
     every substage, filter p=16 alpha=36, gamma 5/3, CFL 0.3) on a band-limited
     random state n^3 with a non-solenoidal B (seeded; the first projection
     removes its longitudinal part): time per step after one warm-up step in
-    FP64 and FP32, max |div B| and relative mass / energy drift (FP64 sums)
-    after the timed steps, then per-kernel times from two more steps."""
+    FP64 and FP32 (the solver's steps take the fused form, its input being
+    solenoidal), max |div B| and relative mass / energy drift (FP64 sums)
+    after the timed steps, then per-kernel times from two more steps; and the
+    step with a projection after every substage (ph_step_mhd3d) timed on the
+    same state, with its distance from the fused step."""
     import torch
     from phrike_amd import _lib
     from phrike_amd.grid import Grid3D
@@ -384,7 +387,8 @@ def mhd_study(n: int, steps: int, dev):
     U0[4] = H[4] + 0.5 * (B ** 2).sum(axis=0)
     del H, Bn, B
     out = {"workload": f"ideal MHD {n}^3, band-limited random state (|m| <= 6), B0 = (0, 0, 1) + 30% "
-                       f"random, {steps} RK4 steps, projection after every substage"}
+                       f"random, {steps} RK4 steps; fused stages and one projection per step "
+                       f"(PH_STEP_SOLENOIDAL) beside a projection after every substage"}
     for prec in ("double", "single"):
         g = Grid3D(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi, dealias=True,
                    filter_params={"enabled": True, "p": 16, "alpha": 36.0}, backend="hip", precision=prec)
@@ -413,7 +417,27 @@ def mhd_study(n: int, steps: int, dev):
         torch.cuda.synchronize()
         _lib.timing_enable(False)
         kt = _lib.timing_collect()
+        # the same steps with a projection after every substage
+        ws, st = _lib.ptr(s._ws), s._st()
+        V, vb = U.clone(), torch.empty_like(U)
+        _lib.call("ph_step_mhd3d", s._plan, _lib.ptr(V), _lib.ptr(vb), dt, ws, st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.call("ph_step_mhd3d", s._plan, _lib.ptr(V), _lib.ptr(vb), dt, ws, st)
+            V, vb = vb, V
+        torch.cuda.synchronize()
+        el_sub = (time.perf_counter() - t0) / steps
+        W = U.clone()
+        s._mark_solenoidal(W)
+        Wf = s.step(W, dt)
+        Ws = torch.empty_like(W)
+        _lib.call("ph_step_mhd3d", s._plan, _lib.ptr(W), _lib.ptr(Ws), dt, ws, st)
+        fd = ((Wf - Ws).double().norm() / Ws.double().norm()).item()
+        del V, vb, W, Wf, Ws
         out[prec] = {"ms_per_step": round(el * 1e3, 3), "mcell_updates_per_s": round(n ** 3 / el / 1e6, 1),
+                     "ms_per_step_substage_projection": round(el_sub * 1e3, 3),
+                     "fused_vs_substage_projection_rel_l2": float(f"{fd:.3e}"),
                      "max_div_b": float(f"{s.max_div_b(U):.3e}"),
                      "mass_drift": float(f"{(c1['mass'] - c0['mass']) / c0['mass']:.3e}"),
                      "energy_drift": float(f"{(c1['energy'] - c0['energy']) / c0['energy']:.3e}"),

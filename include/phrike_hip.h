@@ -366,6 +366,19 @@ int ph_project_mhd3d(ph_planmhd3d_t plan, void* U, void* ws, void* stream);
  * (phrike/solver.py:956-1006, 1037-1050). */
 int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void* ws, void* stream);
 
+/* ph_step_mhd3d with flags.  PH_STEP_SOLENOIDAL: B of U is solenoidal (U is
+ * the output of a previous step, or a projected initial state).  The RHS of B
+ * is then solenoidal as well (the induction flux is antisymmetric and every
+ * spectral factor is common to the three components), so every stage state
+ * keeps a solenoidal B and the projection after each substage is the
+ * identity up to round-off: the step runs its stage boundaries fused, as the
+ * Euler step does, and projects once after the filter.  Results equal the
+ * unflagged step to round-off.  Without the flag: ph_step_mhd3d.  Replaces
+ * the same routines as ph_step_mhd3d (phrike/solver.py:956-1006, 1037-1050). */
+#define PH_STEP_SOLENOIDAL 2
+int ph_step_mhd3d_ex(ph_planmhd3d_t plan, const void* U, void* out, double dt, int flags, void* ws,
+                     void* stream);
+
 /* max(|ux|,|uy|,|uz|) + sqrt(max(a^2 + b^2, 0)) over the grid to max_dev[0]
  * and/or dt_dev[0] = cfl*min(dx,dy,dz)/max (1e-6 if max <= 0).  Replaces
  * MHDEquations3D.max_wave_speed (phrike/equations.py:1041-1061) and

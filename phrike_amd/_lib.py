@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("PHRIKE_HIP_LIB") or os.path.join(os.path.dirname(os.p
 PH_F64 = 0
 PH_F32 = 1
 PH_STEP_BANDLIMITED = 1
+PH_STEP_SOLENOIDAL = 2
 # device-resident run loop scalars (ph_run3d / ph_run2d)
 PH_RS_T, PH_RS_DT, PH_RS_TEND, PH_RS_TSTOP, PH_RS_DTNEXT, PH_RS_DTUSED, PH_RS_N = range(7)
 PH_RF_ACTIVE, PH_RF_STEPS, PH_RF_N = range(3)
@@ -72,6 +73,7 @@ _SIGNATURES = {
     "ph_rhs_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_project_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_step_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "ph_step_mhd3d_ex": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_void_p, c_void_p]),
     "ph_max_wave_speed_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "ph_conserved_sums_mhd3d": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ph_planh_create": (c_int, [POINTER(c_void_p), c_int, c_int, c_double, c_double, c_int, c_int,

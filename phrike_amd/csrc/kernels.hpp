@@ -130,6 +130,10 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #ifndef PH_XMID_PREFETCH
 #define PH_XMID_PREFETCH 1
 #endif
+// the same for the 8 ideal-MHD fields (twice the registers per prefetched field set)
+#ifndef PH_XMID_PREFETCH_MHD
+#define PH_XMID_PREFETCH_MHD 2
+#endif
 // minimum waves per SIMD requested for the x kernels (1: compiler's choice)
 #ifndef PH_X_WAVES_PER_EU
 #define PH_X_WAVES_PER_EU 1
@@ -172,7 +176,9 @@ __device__ __forceinline__ T xquot(T a, T b, T rb) {
 // ----------------------------------------------------------------------------
 // x forward
 // ----------------------------------------------------------------------------
-enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2, XF_MHD3 = 3 };
+enum XFwdMode { XF_FLUX3 = 0, XF_FLUX2 = 1, XF_COPY = 2, XF_MHD3 = 3, XF_COPY8 = 4 };
+// copy modes transform the conserved values themselves (no fluxes)
+constexpr bool xf_copy(int mode) { return mode == XF_COPY || mode == XF_COPY8; }
 
 template <typename T>
 struct XFwdArgs {
@@ -180,7 +186,7 @@ struct XFwdArgs {
   long long in_fs, in_ss;  // field / slab strides (elements); rows of nx
   cx<T>* out;
   long long out_fs, out_ss;  // rows of mxp complex
-  int nf;                    // XF_COPY: number of fields (<= 5)
+  int nf;                    // XF_COPY: number of fields (<= 5; XF_COPY8: <= 8)
   int mx, mxp;
   unsigned ikmask;  // bit f: multiply field f by i*kx (uniform over a group of lines)
   const cx<T>* tw;
@@ -201,6 +207,7 @@ template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
+template <> struct XFwdShape<XF_COPY8> { static constexpr int NC = 8, LX = 8, NG = 1; };
 
 
 // ----------------------------------------------------------------------------
@@ -585,40 +592,40 @@ __device__ __forceinline__ void rk_point(const XInvArgs<T>& a, const T (&v)[NC],
   if constexpr (CLAMP) clamp2d(u, a.gamma, a.rho_min, a.p_min);
 }
 
-// MHD stage combine at one point (phrike/solver.py:988-1006) with k = -v:
-// stage 1: acc = k; stages 2, 3: acc = acc + 2k; next state U + c k (c = dt/2,
-// dt/2, dt); stage 4: U + dt/6 (acc + k).  Then the clamp rho >= rho_min,
-// p >= p_min through prim -> cons (phrike/solver.py:956-964) into u.
+// MHD positivity clamp through prim -> cons (phrike/solver.py:956-964):
+// rho >= rho_min, thermal p >= p_min, velocities and B kept
 template <typename T>
-__device__ __forceinline__ void mhd_stage_point(const XInvArgs<T>& a, const T (&v)[8], long long o, double dtd,
-                                                T (&u)[8]) {
-  const T c = T(a.stage == 3 ? dtd : 0.5 * dtd), sixth = T(dtd / 6.0);
-#pragma unroll
-  for (int f = 0; f < 8; ++f) {
-    const long long idx = f * a.fs + o;
-    const T k = -v[f];
-    const T un = a.un[idx];
-    if (a.stage < 4) {
-      const T acc = a.stage == 1 ? k : a.out[idx] + T(2) * k;
-      a.out[idx] = acc;
-      u[f] = un + c * k;
-    } else {
-      u[f] = un + sixth * (a.out[idx] + k);
-    }
-  }
+__device__ __forceinline__ void clamp_mhd(T (&u)[8], T gamma, T rho_min, T p_min) {
   const T rho = u[0];
   const T ux = u[1] / rho, uy = u[2] / rho, uz = u[3] / rho;
   const T Bx = u[5], By = u[6], Bz = u[7];
   const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
   const T mag = T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-  const T p = (a.gamma - T(1)) * ((u[4] - kin) - mag);
-  const T rc = rho < a.rho_min ? a.rho_min : rho;  // NaN propagates like np.maximum
-  const T pc = p < a.p_min ? a.p_min : p;
+  const T p = (gamma - T(1)) * ((u[4] - kin) - mag);
+  const T rc = rho < rho_min ? rho_min : rho;  // NaN propagates like np.maximum
+  const T pc = p < p_min ? p_min : p;
   u[0] = rc;
   u[1] = rc * ux;
   u[2] = rc * uy;
   u[3] = rc * uz;
-  u[4] = (pc / (a.gamma - T(1)) + (T(0.5) * rc) * ((ux * ux + uy * uy) + uz * uz)) + mag;
+  u[4] = (pc / (gamma - T(1)) + (T(0.5) * rc) * ((ux * ux + uy * uy) + uz * uz)) + mag;
+}
+
+// MHD stage combine at one point (phrike/solver.py:988-1006) with k = -v:
+// stage 1: acc = k; stages 2, 3: acc = acc + 2k; next state U + c k (c = dt/2,
+// dt/2, dt); stage 4: U + dt/6 (acc + k).  Then the clamp rho >= rho_min,
+// p >= p_min through prim -> cons (phrike/solver.py:956-964) into u.
+template <typename T>
+__device__ __forceinline__ void mhd_stage_vals(const XInvArgs<T>& a, const T (&v)[8], const T (&un)[8],
+                                               const T (&acc)[8], double dtd, T (&u)[8], T (&acc_new)[8]) {
+  const T c = T(a.stage == 3 ? dtd : 0.5 * dtd), sixth = T(dtd / 6.0);
+#pragma unroll
+  for (int f = 0; f < 8; ++f) {
+    const T k = -v[f];
+    acc_new[f] = a.stage == 1 ? k : acc[f] + T(2) * k;
+    u[f] = a.stage < 4 ? un[f] + c * k : un[f] + sixth * (acc[f] + k);
+  }
+  clamp_mhd(u, a.gamma, a.rho_min, a.p_min);
 }
 
 // Fused stage boundary: x inverse + RK combine of stage s, then (stages 1-3)
@@ -859,7 +866,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
         w[9] = (ux * Bx + uy * By) + uz * Bz;
         w[10] = u[4] + ps;
       }
-  } else if constexpr (MODE != XF_COPY) {
+  } else if constexpr (!xf_copy(MODE)) {
 #pragma unroll
     for (int j = 0; j < PJ; ++j)
 #pragma unroll
@@ -885,7 +892,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
       }
   }
   cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y * a.mxp;
-  const int nl = MODE == XF_COPY ? ncomp : LX;
+  const int nl = xf_copy(MODE) ? ncomp : LX;
 #pragma unroll 1
   for (int g = 0; g < S::NG; ++g) {
 #pragma unroll
@@ -897,7 +904,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
           T v[2];
 #pragma unroll
           for (int r = 0; r < 2; ++r) {
-            if constexpr (MODE == XF_COPY) {
+            if constexpr (xf_copy(MODE)) {
               v[r] = q[j][r][l];
             } else if constexpr (MODE == XF_MHD3) {
               // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000);
@@ -954,7 +961,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   PH_XTW_FILL(a.tw, NX, NT, tid);
   xr_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
   const T* src = a.in + (long long)slab * a.in_ss + (long long)y * NX;
-  const int ncomp = (MODE == XF_COPY) ? a.nf : NC;
+  const int ncomp = xf_copy(MODE) ? a.nf : NC;
   T q[PJ][2][NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -974,7 +981,9 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
 // RK4 stage combine of the x pair (2p, 2p+1) with k_s = -v (phrike/solver.py:
 // 1220-1230 3D, 502-513 2D); as rk_point: stages 1-3 give the next stage
 // state in u and the updated accumulator in acc_new, stage 4 gives
-// acc + dt/6 k4; the 2D form clamps u (phrike/solver.py:467-488)
+// acc + dt/6 k4 in u and acc_new; the 2D form clamps u (phrike/solver.py:
+// 467-488), the 8-field ideal-MHD form through its own prim -> cons
+// (phrike/solver.py:956-964).  acc_new at stage 4 is the unclamped sum.
 template <typename T, int NC, bool CLAMP>
 __device__ __forceinline__ void rk_pair(const XInvArgs<T>& a, const T (&v)[2][NC], double dtd, const cx<T> (&un)[NC],
                                         const cx<T> (&acc)[NC], T (&u)[2][NC], cx<T> (&acc_new)[NC]) {
@@ -994,8 +1003,13 @@ __device__ __forceinline__ void rk_pair(const XInvArgs<T>& a, const T (&v)[2][NC
     acc_new[c] = last ? mk(u[0][c], u[1][c]) : mk(ba.x + w * k0, ba.y + w * k1);
   }
   if constexpr (CLAMP) {
-    clamp2d(u[0], a.gamma, a.rho_min, a.p_min);
-    clamp2d(u[1], a.gamma, a.rho_min, a.p_min);
+    if constexpr (NC == 8) {
+      clamp_mhd(u[0], a.gamma, a.rho_min, a.p_min);
+      clamp_mhd(u[1], a.gamma, a.rho_min, a.p_min);
+    } else {
+      clamp2d(u[0], a.gamma, a.rho_min, a.p_min);
+      clamp2d(u[1], a.gamma, a.rho_min, a.p_min);
+    }
   }
 }
 
@@ -1074,35 +1088,68 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       xr_store_pairs<G>(row(a.out, c), tid, o);
     }
     if (MODE != XI_RHS && a.cfl_rows != nullptr) cfl_store(v);
-  } else if constexpr (MODE == XI_PROJE || MODE == XI_MHDSTAGE) {
+  } else if constexpr (MODE == XI_PROJE) {
+    // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933:
+    // B <- the projected B, E at the thermal pressure of the unprojected state
+    cx<T> w[8][PJ];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) xr_load_pairs<G>(row(a.out, f), tid, w[f]);
 #pragma unroll
     for (int j = 0; j < PJ; ++j) {
-      const int p = G::point(tid, j);
-      if (!G::PALL && p >= G::M) continue;
+      T e[2], b[2][3];
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
-        const long long o = base + 2 * p + r;
-        if constexpr (MODE == XI_PROJE) {
-          // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933
-          T* u = a.out;
-          const T rho = u[o];
-          const T ux = u[a.fs + o] / rho, uy = u[2 * a.fs + o] / rho, uz = u[3 * a.fs + o] / rho;
-          const T b0 = u[5 * a.fs + o], b1 = u[6 * a.fs + o], b2 = u[7 * a.fs + o];
-          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-          const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
-          const T pp = (a.gamma - T(1)) * ((u[4 * a.fs + o] - kin) - mag);
-          const T Bx = v[j][r][0], By = v[j][r][1], Bz = v[j][r][2];
-          u[4 * a.fs + o] = (pp / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-          u[5 * a.fs + o] = Bx;
-          u[6 * a.fs + o] = By;
-          u[7 * a.fs + o] = Bz;
-        } else {
-          T u[8];
-          mhd_stage_point<T>(a, v[j][r], o, dtd, u);
-#pragma unroll
-          for (int c = 0; c < 8; ++c) a.us[c * a.fs + o] = u[c];
-        }
+        auto at = [&](int f) { return r == 0 ? w[f][j].x : w[f][j].y; };
+        const T rho = at(0);
+        const T ux = at(1) / rho, uy = at(2) / rho, uz = at(3) / rho;
+        const T b0 = at(5), b1 = at(6), b2 = at(7);
+        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+        const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
+        const T pp = (a.gamma - T(1)) * ((at(4) - kin) - mag);
+        const T Bx = v[j][r][0], By = v[j][r][1], Bz = v[j][r][2];
+        e[r] = (pp / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+        b[r][0] = Bx;
+        b[r][1] = By;
+        b[r][2] = Bz;
       }
+      w[4][j] = mk(e[0], e[1]);
+#pragma unroll
+      for (int f = 0; f < 3; ++f) w[5 + f][j] = mk(b[0][f], b[1][f]);
+    }
+#pragma unroll
+    for (int f = 4; f < 8; ++f) xr_store_pairs<G>(row(a.out, f), tid, w[f]);
+  } else if constexpr (MODE == XI_MHDSTAGE) {
+    cx<T> un[8][PJ], acc[8][PJ];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      xr_load_pairs<G>(row(const_cast<T*>(a.un), f), tid, un[f]);
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) acc[f][j] = mk(T(0), T(0));
+      if (a.stage > 1) xr_load_pairs<G>(row(a.out, f), tid, acc[f]);
+    }
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      T u[2][8], an[2][8];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        T unr[8], accr[8];
+#pragma unroll
+        for (int f = 0; f < 8; ++f) {
+          unr[f] = r == 0 ? un[f][j].x : un[f][j].y;
+          accr[f] = r == 0 ? acc[f][j].x : acc[f][j].y;
+        }
+        mhd_stage_vals<T>(a, v[j][r], unr, accr, dtd, u[r], an[r]);
+      }
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        un[f][j] = mk(u[0][f], u[1][f]);  // the next state
+        acc[f][j] = mk(an[0][f], an[1][f]);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 8; ++f) {
+      if (a.stage < 4) xr_store_pairs<G>(row(a.out, f), tid, acc[f]);
+      xr_store_pairs<G>(row(a.us, f), tid, un[f]);
     }
   } else {
     cx<T> un[NC][PJ], acc[NC][PJ];
@@ -1166,9 +1213,12 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   using G = XRGeo<T, NX, DA>;
   constexpr int NT = G::NT, PJ = G::PJ;
   static_assert(XFwdShape<FMODE>::NC == NC || FMODE == XF_COPY, "field counts");
+  // stage 4 transforms the new state itself: 5 fields (Euler; 4 in 2D) or 8 (MHD)
+  constexpr int LMODE = NC == 8 ? XF_COPY8 : XF_COPY;
+  constexpr int QN = XFwdShape<LMODE>::NC;
   PH_XTW_DECL(T, NX);
   constexpr int LB0 = NC > XFwdShape<FMODE>::LX ? NC : XFwdShape<FMODE>::LX;
-  constexpr int LB = LB0 > XFwdShape<XF_COPY>::LX ? LB0 : XFwdShape<XF_COPY>::LX;
+  constexpr int LB = LB0 > XFwdShape<LMODE>::LX ? LB0 : XFwdShape<LMODE>::LX;
   __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const XInvArgs<T>& a = m.inv;
@@ -1185,7 +1235,8 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   // the RK combine's inputs (x pairs), per field: U (PH_XMID_PREFETCH 2), or
   // U and the accumulator (1), loaded before the inverse transform so their
   // latency overlaps it; the rest after it
-  constexpr bool PRE_UN = PH_XMID_PREFETCH != 0, PRE_ACC = PH_XMID_PREFETCH == 1;
+  constexpr int PF = NC == 8 ? PH_XMID_PREFETCH_MHD : PH_XMID_PREFETCH;
+  constexpr bool PRE_UN = PF != 0, PRE_ACC = PF == 1;
   auto row = [&](T* f, int c) { return f + c * a.fs + base; };
   cx<T> un[NC][PJ], acc[NC][PJ];
 #pragma unroll
@@ -1221,7 +1272,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
     }
   }
-  T q[PJ][2][5];
+  T q[PJ][2][QN];
   cx<T> an[NC][PJ];
 #pragma unroll
   for (int j = 0; j < PJ; ++j) {
@@ -1240,15 +1291,17 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       un_j[c] = un[c][j];
       acc_j[c] = acc[c][j];
     }
-    rk_pair<T, NC, NC == 4>(a, v, dtd, un_j, acc_j, u, an_j);
+    rk_pair<T, NC, NC == 4 || NC == 8>(a, v, dtd, un_j, acc_j, u, an_j);
 #pragma unroll
     for (int c = 0; c < NC; ++c) an[c][j] = an_j[c];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) q[j][r][c] = c < NC ? u[r][c] : T(0);
+      for (int c = 0; c < QN; ++c) q[j][r][c] = c < NC ? u[r][c] : T(0);
   }
-  if (a.stage < 4 || a.keep_state) {
+  // keep_state (stage 4 also writes the new state) is an option of the
+  // unclamped 3D Euler form only, where the stage-4 accumulator is the state
+  if (a.stage < 4 || (NC == 5 && a.keep_state)) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, an[c]);
   }
@@ -1263,7 +1316,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
         for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
     xr_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, qf);
   } else {
-    xr_forward<T, NX, DA, XF_COPY>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
+    xr_forward<T, NX, DA, LMODE>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
   }
 }
 

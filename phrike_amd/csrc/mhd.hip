@@ -17,6 +17,8 @@
 //          and E = p/(gamma-1) + KE + |B|^2/2 at the thermal pressure p of the
 //          unprojected state.
 //   filter the main plan's spectral filter of all 8 fields.
+//   fused  for a solenoidal U (PH_STEP_SOLENOIDAL): the stage boundaries as
+//          one k_xmid each, one projection per step (step_fused).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -280,6 +282,55 @@ struct PlanMHD3D : PlanBase {
       main.deriv(-1, nf, U + f0 * ncell, U + f0 * ncell, wsMain(ws), st);
     }
   }
+  // Fused step for a state whose B is solenoidal (the output of a previous
+  // step, or a projected initial state).  The ideal-MHD induction flux is
+  // antisymmetric, F_g[B_i] = u_g B_i - u_i B_g, and every spectral factor of
+  // the RHS (i k_g, the 2/3 mask, sigma) is the same for all components, so
+  // sum_i k_i rhs^_B,i = -i sigma sum_{g,i} k_g k_i F^_g,i = 0: the RHS of B is
+  // solenoidal, every stage state U + c k keeps the solenoidal B of U (the
+  // clamp leaves B alone), and the projection after each substage is the
+  // identity up to round-off.  The stages therefore run as the Euler
+  // pipeline does: one k_xmid per stage boundary (x inverse, RK4 accumulator
+  // combine, clamp, next fluxes and x forward), stage states never in HBM;
+  // then the spectral filter of the 8 fields and one projection, which
+  // removes the round-off divergence (phrike/solver.py:988-1006, 1041-1050).
+  void step_fused(const T* U, T* out, double dt, void* ws, hipStream_t st) const {
+    void* w = wsMain(ws);
+    main.xfwd(XF_MHD3, U, 8, 0xFFu, w, st);
+    for (int s = 1; s <= 4; ++s) {
+      main.yfwd(true, 8, 0, w, st);
+      main.zpass(true, 8, 0, w, st);
+      main.yinv(8, w, st);
+      XMidArgs<T> m{};
+      m.inv = main.xinv_args(w);
+      m.inv.nf = 8;
+      m.inv.out = out;  // the RK4 accumulator until stage 4
+      m.inv.un = U;
+      m.inv.stage = s;
+      m.inv.dt = dt;
+      m.inv.rho_min = T(MHD_RHO_MIN);
+      m.inv.p_min = T(MHD_P_MIN);
+      m.fwd = main.xfwd_args(nullptr, 8, 0xFFu, w);
+      m.fwd_last = main.xfwd_args(nullptr, 8, 0u, w);
+      KScope ks(s == 4 ? "xmid_mhd_last" : "xmid_mhd", st);
+      launch_xmid<T, 8, XF_MHD3>(nx, ny, nz, m, st);
+      check_launch("k_xmid(mhd)");
+    }
+    // the filter's y, z and y-inverse passes of the 8 fields; the x inverse
+    // writes the filtered state
+    main.yfwd(false, 8, 0, w, st);
+    main.zpass(false, 8, 0, w, st);
+    main.yinv(8, w, st);
+    {
+      XInvArgs<T> a = main.xinv_args(w);
+      a.nf = 8;
+      a.out = out;
+      KScope ks("xinv_out8", st);
+      launch_xinv<T, 8, XI_OUT>(nx, ny, nz, a, st);
+      check_launch("k_xinv(mhd filter)");
+    }
+    project(out, ws, st);
+  }
   // phrike/solver.py:988-1006, 1041-1050
   void step(const T* U, T* out, double dt, void* ws, hipStream_t st) const {
     T* us = Us(ws);
@@ -405,16 +456,23 @@ int ph_project_mhd3d(ph_planmhd3d_t plan, void* U, void* ws, void* stream) {
   });
 }
 
-int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void* ws, void* stream) {
+int ph_step_mhd3d_ex(ph_planmhd3d_t plan, const void* U, void* out, double dt, int flags, void* ws,
+                     void* stream) {
   return mguard([&] {
     if (!U || !out || !ws) throw ph::Error(PH_ERR_ARG, "null buffer");
+    if (flags & ~PH_STEP_SOLENOIDAL) throw ph::Error(PH_ERR_ARG, "unknown step flags");
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;
       const size_t n = 8ull * size_t(p.ncell) * sizeof(R);
       if (ph::ranges_overlap(U, n, out, n)) throw ph::Error(PH_ERR_ARG, "U and out must not overlap");
-      p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, ws, MS(stream));
+      if (flags & PH_STEP_SOLENOIDAL) p.step_fused(static_cast<const R*>(U), static_cast<R*>(out), dt, ws, MS(stream));
+      else p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, ws, MS(stream));
     });
   });
+}
+
+int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void* ws, void* stream) {
+  return ph_step_mhd3d_ex(plan, U, out, dt, 0, ws, stream);
 }
 
 int ph_max_wave_speed_mhd3d(ph_planmhd3d_t plan, const void* U, double* max_dev, double* dt_dev, double cfl,

@@ -4,14 +4,18 @@ MHDEquations3D and SpectralSolverMHD3D keep the reference's names, constructor
 arguments and step / compute_dt / run / max_div_b contract
 (phrike/equations.py:911-1100, phrike/solver.py:840-1163, 1173).  The state is
 (8, Nz, Ny, Nx) on the GPU: rho, rho ux, rho uy, rho uz, E, Bx, By, Bz.  A step
-is one C-ABI call (ph_step_mhd3d): four RHS evaluations on the pencil pipeline
-with 8 fields, the clamp and the Helmholtz projection of B after every
-substage, then the spectral filter and the projection.  The runtime options
+is one C-ABI call (ph_step_mhd3d_ex): four RHS evaluations on the pencil
+pipeline with 8 fields, the clamp and the Helmholtz projection of B after every
+substage, then the spectral filter and the projection.  When B of the input is
+known to be solenoidal (the solver's own previous output, or its projected
+initial state) the step says so (PH_STEP_SOLENOIDAL): the substage projections
+are then the identity up to round-off and the library runs the fused form.  The runtime options
 are read from ``grid.mhd_config`` as in the reference; only its defaults
 (no resistivity or viscosity, projection after every substage) are built.
 """
 from __future__ import annotations
 
+import weakref
 from typing import Any, Callable, Dict, List, Optional
 
 import numpy as np
@@ -94,6 +98,16 @@ class SpectralSolverMHD3D:
         _lib.call("ph_planmhd3d_workspace_bytes", h, _lib.ctypes.byref(n))
         self._ws = torch.empty(n.value, dtype=torch.uint8, device=grid.device)
         self._scalars = torch.zeros(8, dtype=torch.float64, device=grid.device)
+        self._sol_ref = None  # weak reference to the last tensor known to hold a solenoidal B
+        self._sol_version = -1
+
+    def _mark_solenoidal(self, T) -> None:
+        self._sol_ref = weakref.ref(T)
+        self._sol_version = T._version
+
+    def _is_solenoidal(self, U) -> bool:
+        t = self._sol_ref() if self._sol_ref is not None else None
+        return t is U and U._version == self._sol_version
 
     def __del__(self):
         try:
@@ -125,6 +139,7 @@ class SpectralSolverMHD3D:
         returns a new tensor (phrike/solver.py:966-986)."""
         V = self._state(U).clone()
         _lib.call("ph_project_mhd3d", self._plan, _lib.ptr(V), _lib.ptr(self._ws), self._st())
+        self._mark_solenoidal(V)
         return V
 
     def max_div_b(self, U) -> float:
@@ -168,12 +183,17 @@ class SpectralSolverMHD3D:
 
     def step_into(self, U, out, dt: float) -> None:
         """out = one step of U with caller-owned output (checked: same shape,
-        dtype and device as U, contiguous, no shared memory)."""
+        dtype and device as U, contiguous, no shared memory).  When U is this
+        solver's own previous output or projected state, unmodified since
+        (same tensor object, same version counter), the step takes the fused
+        form (PH_STEP_SOLENOIDAL)."""
         from .solver import _check_pair
         U = self._state(U)
         _check_pair(U, out, "out")
-        _lib.call("ph_step_mhd3d", self._plan, _lib.ptr(U), _lib.ptr(out), float(dt), _lib.ptr(self._ws),
-                  self._st())
+        flags = _lib.PH_STEP_SOLENOIDAL if self._is_solenoidal(U) else 0
+        _lib.call("ph_step_mhd3d_ex", self._plan, _lib.ptr(U), _lib.ptr(out), float(dt), flags,
+                  _lib.ptr(self._ws), self._st())
+        self._mark_solenoidal(out)
 
     def run(self, U0, t0: float, t_end: float, output_interval: float = 0.1,
             checkpoint_interval: float = 0.0, outdir: Optional[str] = None,

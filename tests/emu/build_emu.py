@@ -4,6 +4,7 @@ host sources compiled by g++ against tests/emu/hip/hip_runtime.h.
     python tests/emu/build_emu.py [--asan]
 """
 import os
+import shutil
 import subprocess
 import sys
 
@@ -16,7 +17,10 @@ SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "hybrid.hip", "mhd.hip", 
 
 
 def build(asan: bool = False) -> str:
-    os.makedirs(OUT, exist_ok=True)
+    # objects in a per-process folder and an atomic rename of the library, so
+    # parallel test workers building at once never load a half-written file
+    work = os.path.join(OUT, f"p{os.getpid()}")
+    os.makedirs(work, exist_ok=True)
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-I", HERE, "-Wno-unused-result"]
     # extra defines for tuning variants of the kernels, e.g. EMU_DEFINES="-DPH_XPADB=4"
     flags += os.environ.get("EMU_DEFINES", "").split()
@@ -25,7 +29,7 @@ def build(asan: bool = False) -> str:
     objs = []
     procs = []
     for s in SOURCES:
-        o = os.path.join(OUT, os.path.splitext(s)[0] + ".o")
+        o = os.path.join(work, os.path.splitext(s)[0] + ".o")
         objs.append(o)
         src = os.path.join(HERE, s) if s.endswith(".cpp") else os.path.join(CSRC, s)
         procs.append(subprocess.Popen(["g++", *flags, "-x", "c++", "-c", src, "-o", o],
@@ -35,9 +39,12 @@ def build(asan: bool = False) -> str:
         if p.returncode != 0:
             raise RuntimeError(out)
     lib = LIB.replace(".so", "_asan.so") if asan else LIB
-    r = subprocess.run(["g++", *flags, "-shared", "-o", lib, *objs, "-lpthread"], capture_output=True, text=True)
+    tmp = os.path.join(work, os.path.basename(lib))
+    r = subprocess.run(["g++", *flags, "-shared", "-o", tmp, *objs, "-lpthread"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stdout + r.stderr)
+    os.replace(tmp, lib)
+    shutil.rmtree(work, ignore_errors=True)
     return lib
 
 

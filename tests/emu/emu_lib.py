@@ -178,10 +178,10 @@ class EmuMHD3D:
         self._chk(self.lib.ph_project_mhd3d(self.h, P(V), P(self.ws), None))
         return V
 
-    def step(self, U, dt):
+    def step(self, U, dt, flags=0):
         U = np.ascontiguousarray(U, dtype=self.dtype)
         out = np.zeros_like(U)
-        self._chk(self.lib.ph_step_mhd3d(self.h, P(U), P(out), dt, P(self.ws), None))
+        self._chk(self.lib.ph_step_mhd3d_ex(self.h, P(U), P(out), dt, flags, P(self.ws), None))
         return out
 
     def max_speed_dt(self, U, cfl):

@@ -125,6 +125,33 @@ def test_mhd_kernels_vs_reference(emu):
     assert field_errors(a.step(d["alf_U0"], float(d["alf_dt"])), d["alf_U1"]).max() < 1e-11
 
 
+def test_mhd_fused_step(emu):
+    """The fused MHD step (PH_STEP_SOLENOIDAL: stage boundaries in k_xmid, one
+    projection per step) on a solenoidal state equals the step with a
+    projection after every substage to round-off, and continues the
+    reference's own runs: random 16^3 steps 2-3 after an unfused first step
+    (which projects its non-solenoidal input), the Alfven wave's 5 steps."""
+    from emu_lib import EmuMHD3D
+    from phrike_amd import _lib
+    SOL = _lib.PH_STEP_SOLENOIDAL
+    d = golden("mhd3d.npz")
+    e = EmuMHD3D(emu, 16, 16, 16)
+    dt = float(d["rnd_dt"])
+    V0 = e.project(d["rnd_U"])
+    assert field_errors(e.step(V0, dt, SOL), e.step(V0, dt)).max() < 1e-13
+    W = e.step(d["rnd_U"], dt)
+    for _ in range(2):
+        W = e.step(W, dt, SOL)
+    assert field_errors(W, d["rnd_U3"]).max() < 1e-11
+    a = EmuMHD3D(emu, 32, 8, 8, L=(1.0, 1.0, 1.0), filt=(1, 8, 36.0))
+    A = d["alf_U0"]
+    for _ in range(5):
+        A = a.step(A, float(d["alf_dt"]), SOL)
+    assert field_errors(A, d["alf_U5"]).max() < 1e-11
+    with pytest.raises(RuntimeError, match="flags"):
+        e.step(V0, dt, 4)
+
+
 def test_plan_kind_checked(emu):
     """A plan handle of another kind is an argument error, not undefined
     behaviour: every entry point checks the plan's dynamic type."""
