@@ -83,3 +83,30 @@ def test_oracle_mhd_closed_forms():
     for a, b in zip(Fm, Fe):
         np.testing.assert_allclose(a[:5], b, rtol=1e-15, atol=1e-15)
         assert np.all(a[5:] == 0)
+
+
+def test_rhs_of_b_is_solenoidal():
+    """The premise of the fused MHD step: the ideal-MHD RHS of B has zero
+    spectral divergence (the induction flux is antisymmetric and every
+    spectral factor is common to the components), even for a state whose own
+    B is not solenoidal.  Checked on the reference's own RHS golden and on the
+    oracle, against the divergence of B itself (which is far from zero); and a
+    stage state U + c rhs of a projected U is left unchanged by the projection
+    up to round-off."""
+    from oracle import mhd_np as mn
+    d = golden("mhd3d.npz")
+    alf, rnd = _grids()
+    e = mn.MHD3DOracle(5.0 / 3.0)
+    U = d["rnd_U"]
+    div_b = np.abs(mn.divergence(rnd, U[5], U[6], U[7])).max()
+    assert div_b > 1e-3
+    R = d["rnd_rhs"]
+    scale = np.abs(R[5:]).max() * (16 // 3)  # |rhs_B| times the largest kept |k|
+    div_ref = np.abs(mn.divergence(rnd, R[5], R[6], R[7])).max()
+    div_orc = np.abs(mn.divergence(rnd, *mn.rhs_mhd3d(rnd, e, U)[5:])).max()
+    assert div_ref < 1e-13 * scale and div_orc < 1e-13 * scale, (div_ref, div_orc, scale)
+    s = mn.SolverMHD3DOracle(rnd, e, cfl=0.3)
+    V = s.project_B(U)
+    S = V + 0.01 * mn.rhs_mhd3d(rnd, e, V)
+    P = np.array(mn.project_solenoidal(rnd, S[5], S[6], S[7]))
+    assert field_errors(P, S[5:]).max() < 1e-14
