@@ -134,6 +134,11 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #ifndef PH_XMID_PREFETCH_MHD
 #define PH_XMID_PREFETCH_MHD 2
 #endif
+// rows of one slab per x-kernel block (1 or 2; every axis length is a
+// multiple of 2)
+#ifndef PH_X_ROWS
+#define PH_X_ROWS 1
+#endif
 // minimum waves per SIMD requested for the x kernels (1: compiler's choice)
 #ifndef PH_X_WAVES_PER_EU
 #define PH_X_WAVES_PER_EU 1
@@ -957,25 +962,31 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (gated_off(a.active, a.act_all, slab)) return;
-  const int y = blockIdx.x;
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  xr_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
-  const T* src = a.in + (long long)slab * a.in_ss + (long long)y * NX;
-  const int ncomp = xf_copy(MODE) ? a.nf : NC;
-  T q[PJ][2][NC];
+  // PH_X_ROWS consecutive rows per block; the twiddle and mode tables are
+  // filled once
+#pragma unroll 1
+  for (int rr = 0; rr < PH_X_ROWS; ++rr) {
+    const int y = blockIdx.x * PH_X_ROWS + rr;
+    if (rr > 0) lds_barrier();  // the previous row is done with buf
+    if (rr == 0) PH_XTW_FILL(a.tw, NX, NT, tid);
+    if (rr == 0) xr_tables<T, NX, DA>(a.sig, a.kk, s_sig, s_k, tid);
+    const T* src = a.in + (long long)slab * a.in_ss + (long long)y * NX;
+    const int ncomp = xf_copy(MODE) ? a.nf : NC;
+    T q[PJ][2][NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    cx<T> v[PJ];
+    for (int c = 0; c < NC; ++c) {
+      cx<T> v[PJ];
 #pragma unroll
-    for (int j = 0; j < PJ; ++j) v[j] = mk(T(0), T(0));
-    if (c < ncomp) xr_load_pairs<G>(src + (long long)c * a.in_fs, tid, v);
+      for (int j = 0; j < PJ; ++j) v[j] = mk(T(0), T(0));
+      if (c < ncomp) xr_load_pairs<G>(src + (long long)c * a.in_fs, tid, v);
 #pragma unroll
-    for (int j = 0; j < PJ; ++j) {
-      q[j][0][c] = v[j].x;
-      q[j][1][c] = v[j].y;
+      for (int j = 0; j < PJ; ++j) {
+        q[j][0][c] = v[j].x;
+        q[j][1][c] = v[j].y;
+      }
     }
+    xr_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y, q);
   }
-  xr_forward<T, NX, DA, MODE>(a, ncomp, tw, buf, s_sig, s_k, tid, slab, y, q);
 }
 
 // RK4 stage combine of the x pair (2p, 2p+1) with k_s = -v (phrike/solver.py:
@@ -1033,172 +1044,178 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (gated_off(a.active, a.act_all, slab)) return;
-  const int y = blockIdx.x;
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
-  static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
-  static_assert(MODE != XI_MHDSTAGE || NC == 8, "the MHD stage combine takes the 8 MHD fields");
-  XRSpec<T, NX, DA, NC> sp;
-  sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, nfl, tid);
-  xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
-  const long long base = (long long)slab * a.ss + (long long)y * NX;
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
-  // the transformed values at the thread's x pairs: v[j][r][c] at x = 2p + r
-  T v[PJ][2][NC];
-#pragma unroll
-  for (int j = 0; j < PJ; ++j) {
-    const int p = G::point(tid, j);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
-      v[j][0][c] = z.x;
-      v[j][1][c] = z.y;
-    }
-  }
-  auto row = [&](T* f, int c) { return f + c * a.fs + base; };
-  // the fused CFL reduction of a final state: max wave speed of the values
-  // this thread wrote (u[j][r][c] at x = 2p + r), one partial per row
-  constexpr bool EULER = NC == 4 || NC == 5;
-  auto cfl_store = [&](const T (&u)[PJ][2][NC]) {
-    if constexpr (EULER) {
-      double m = 0.0;
-#pragma unroll
-      for (int j = 0; j < PJ; ++j) {
-        if (!G::PALL && G::point(tid, j) >= G::M) continue;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) m = dmax_nan(m, wave_speed_v<T, NC - 2>(u[j][r], a.gamma));
-      }
-      block_max_store<NT>(m, a.cfl_rows + (long long)slab * gridDim.x + y, tid);
-    }
-  };
-  if constexpr (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (c >= nfl) continue;
-      cx<T> o[PJ];
-      if constexpr (MODE == XI_ADD) xr_load_pairs<G>(row(a.out, c), tid, o);
-#pragma unroll
-      for (int j = 0; j < PJ; ++j) {
-        const T s0 = MODE == XI_RHS ? -v[j][0][c] : v[j][0][c];
-        const T s1 = MODE == XI_RHS ? -v[j][1][c] : v[j][1][c];
-        o[j] = MODE == XI_ADD ? mk(o[j].x + s0, o[j].y + s1) : mk(s0, s1);
-        v[j][0][c] = o[j].x;  // the values written
-        v[j][1][c] = o[j].y;
-      }
-      xr_store_pairs<G>(row(a.out, c), tid, o);
-    }
-    if (MODE != XI_RHS && a.cfl_rows != nullptr) cfl_store(v);
-  } else if constexpr (MODE == XI_PROJE) {
-    // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933:
-    // B <- the projected B, E at the thermal pressure of the unprojected state
-    cx<T> w[8][PJ];
-#pragma unroll
-    for (int f = 0; f < 8; ++f) xr_load_pairs<G>(row(a.out, f), tid, w[f]);
+  // PH_X_ROWS consecutive rows per block; the twiddle and mode tables are
+  // filled once
+#pragma unroll 1
+  for (int rr = 0; rr < PH_X_ROWS; ++rr) {
+    const int y = blockIdx.x * PH_X_ROWS + rr;
+    if (rr > 0) lds_barrier();  // the previous row is done with buf
+    if (rr == 0) PH_XTW_FILL(a.tw, NX, NT, tid);
+    const int nfl = (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) ? a.nf : NC;
+    static_assert(MODE != XI_PROJE || NC == 3, "projection write-back takes the 3 B fields");
+    static_assert(MODE != XI_MHDSTAGE || NC == 8, "the MHD stage combine takes the 8 MHD fields");
+    XRSpec<T, NX, DA, NC> sp;
+    sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, nfl, tid);
+    xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
+    const long long base = (long long)slab * a.ss + (long long)y * NX;
+    const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
+    // the transformed values at the thread's x pairs: v[j][r][c] at x = 2p + r
+    T v[PJ][2][NC];
 #pragma unroll
     for (int j = 0; j < PJ; ++j) {
-      T e[2], b[2][3];
+      const int p = G::point(tid, j);
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        auto at = [&](int f) { return r == 0 ? w[f][j].x : w[f][j].y; };
-        const T rho = at(0);
-        const T ux = at(1) / rho, uy = at(2) / rho, uz = at(3) / rho;
-        const T b0 = at(5), b1 = at(6), b2 = at(7);
-        const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
-        const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
-        const T pp = (a.gamma - T(1)) * ((at(4) - kin) - mag);
-        const T Bx = v[j][r][0], By = v[j][r][1], Bz = v[j][r][2];
-        e[r] = (pp / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
-        b[r][0] = Bx;
-        b[r][1] = By;
-        b[r][2] = Bz;
+      for (int c = 0; c < NC; ++c) {
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        v[j][0][c] = z.x;
+        v[j][1][c] = z.y;
       }
-      w[4][j] = mk(e[0], e[1]);
-#pragma unroll
-      for (int f = 0; f < 3; ++f) w[5 + f][j] = mk(b[0][f], b[1][f]);
     }
+    auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+    // the fused CFL reduction of a final state: max wave speed of the values
+    // this thread wrote (u[j][r][c] at x = 2p + r), one partial per row
+    constexpr bool EULER = NC == 4 || NC == 5;
+    auto cfl_store = [&](const T (&u)[PJ][2][NC]) {
+      if constexpr (EULER) {
+        double m = 0.0;
 #pragma unroll
-    for (int f = 4; f < 8; ++f) xr_store_pairs<G>(row(a.out, f), tid, w[f]);
-  } else if constexpr (MODE == XI_MHDSTAGE) {
-    cx<T> un[8][PJ], acc[8][PJ];
+        for (int j = 0; j < PJ; ++j) {
+          if (!G::PALL && G::point(tid, j) >= G::M) continue;
 #pragma unroll
-    for (int f = 0; f < 8; ++f) {
-      xr_load_pairs<G>(row(const_cast<T*>(a.un), f), tid, un[f]);
+          for (int r = 0; r < 2; ++r) m = dmax_nan(m, wave_speed_v<T, NC - 2>(u[j][r], a.gamma));
+        }
+        block_max_store<NT>(m, a.cfl_rows + (long long)slab * (gridDim.x * PH_X_ROWS) + y, tid);
+      }
+    };
+    if constexpr (MODE == XI_OUT || MODE == XI_RHS || MODE == XI_ADD) {
 #pragma unroll
-      for (int j = 0; j < PJ; ++j) acc[f][j] = mk(T(0), T(0));
-      if (a.stage > 1) xr_load_pairs<G>(row(a.out, f), tid, acc[f]);
-    }
+      for (int c = 0; c < NC; ++c) {
+        if (c >= nfl) continue;
+        cx<T> o[PJ];
+        if constexpr (MODE == XI_ADD) xr_load_pairs<G>(row(a.out, c), tid, o);
 #pragma unroll
-    for (int j = 0; j < PJ; ++j) {
-      T u[2][8], an[2][8];
+        for (int j = 0; j < PJ; ++j) {
+          const T s0 = MODE == XI_RHS ? -v[j][0][c] : v[j][0][c];
+          const T s1 = MODE == XI_RHS ? -v[j][1][c] : v[j][1][c];
+          o[j] = MODE == XI_ADD ? mk(o[j].x + s0, o[j].y + s1) : mk(s0, s1);
+          v[j][0][c] = o[j].x;  // the values written
+          v[j][1][c] = o[j].y;
+        }
+        xr_store_pairs<G>(row(a.out, c), tid, o);
+      }
+      if (MODE != XI_RHS && a.cfl_rows != nullptr) cfl_store(v);
+    } else if constexpr (MODE == XI_PROJE) {
+      // phrike/solver.py:966-986 with the primitives of phrike/equations.py:922-933:
+      // B <- the projected B, E at the thermal pressure of the unprojected state
+      cx<T> w[8][PJ];
 #pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        T unr[8], accr[8];
+      for (int f = 0; f < 8; ++f) xr_load_pairs<G>(row(a.out, f), tid, w[f]);
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        T e[2], b[2][3];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          auto at = [&](int f) { return r == 0 ? w[f][j].x : w[f][j].y; };
+          const T rho = at(0);
+          const T ux = at(1) / rho, uy = at(2) / rho, uz = at(3) / rho;
+          const T b0 = at(5), b1 = at(6), b2 = at(7);
+          const T kin = (T(0.5) * rho) * ((ux * ux + uy * uy) + uz * uz);
+          const T mag = T(0.5) * ((b0 * b0 + b1 * b1) + b2 * b2);
+          const T pp = (a.gamma - T(1)) * ((at(4) - kin) - mag);
+          const T Bx = v[j][r][0], By = v[j][r][1], Bz = v[j][r][2];
+          e[r] = (pp / (a.gamma - T(1)) + kin) + T(0.5) * ((Bx * Bx + By * By) + Bz * Bz);
+          b[r][0] = Bx;
+          b[r][1] = By;
+          b[r][2] = Bz;
+        }
+        w[4][j] = mk(e[0], e[1]);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) w[5 + f][j] = mk(b[0][f], b[1][f]);
+      }
+#pragma unroll
+      for (int f = 4; f < 8; ++f) xr_store_pairs<G>(row(a.out, f), tid, w[f]);
+    } else if constexpr (MODE == XI_MHDSTAGE) {
+      cx<T> un[8][PJ], acc[8][PJ];
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        xr_load_pairs<G>(row(const_cast<T*>(a.un), f), tid, un[f]);
+#pragma unroll
+        for (int j = 0; j < PJ; ++j) acc[f][j] = mk(T(0), T(0));
+        if (a.stage > 1) xr_load_pairs<G>(row(a.out, f), tid, acc[f]);
+      }
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        T u[2][8], an[2][8];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          T unr[8], accr[8];
+#pragma unroll
+          for (int f = 0; f < 8; ++f) {
+            unr[f] = r == 0 ? un[f][j].x : un[f][j].y;
+            accr[f] = r == 0 ? acc[f][j].x : acc[f][j].y;
+          }
+          mhd_stage_vals<T>(a, v[j][r], unr, accr, dtd, u[r], an[r]);
+        }
 #pragma unroll
         for (int f = 0; f < 8; ++f) {
-          unr[f] = r == 0 ? un[f][j].x : un[f][j].y;
-          accr[f] = r == 0 ? acc[f][j].x : acc[f][j].y;
+          un[f][j] = mk(u[0][f], u[1][f]);  // the next state
+          acc[f][j] = mk(an[0][f], an[1][f]);
         }
-        mhd_stage_vals<T>(a, v[j][r], unr, accr, dtd, u[r], an[r]);
       }
 #pragma unroll
       for (int f = 0; f < 8; ++f) {
-        un[f][j] = mk(u[0][f], u[1][f]);  // the next state
-        acc[f][j] = mk(an[0][f], an[1][f]);
+        if (a.stage < 4) xr_store_pairs<G>(row(a.out, f), tid, acc[f]);
+        xr_store_pairs<G>(row(a.us, f), tid, un[f]);
       }
-    }
+    } else {
+      cx<T> un[NC][PJ], acc[NC][PJ];
 #pragma unroll
-    for (int f = 0; f < 8; ++f) {
-      if (a.stage < 4) xr_store_pairs<G>(row(a.out, f), tid, acc[f]);
-      xr_store_pairs<G>(row(a.us, f), tid, un[f]);
-    }
-  } else {
-    cx<T> un[NC][PJ], acc[NC][PJ];
+      for (int c = 0; c < NC; ++c) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
+        for (int j = 0; j < PJ; ++j) {
+          un[c][j] = mk(T(0), T(0));
+          acc[c][j] = mk(T(0), T(0));
+        }
+        if (a.stage < 4) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+        if (a.stage > 1) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+      }
+      cx<T> an[NC][PJ], us[NC][PJ];
 #pragma unroll
       for (int j = 0; j < PJ; ++j) {
-        un[c][j] = mk(T(0), T(0));
-        acc[c][j] = mk(T(0), T(0));
-      }
-      if (a.stage < 4) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
-      if (a.stage > 1) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
-    }
-    cx<T> an[NC][PJ], us[NC][PJ];
-#pragma unroll
-    for (int j = 0; j < PJ; ++j) {
-      cx<T> un_j[NC], acc_j[NC], an_j[NC];
-      T u[2][NC];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        un_j[c] = un[c][j];
-        acc_j[c] = acc[c][j];
-      }
-      rk_pair<T, NC, MODE == XI_STAGE2>(a, v[j], dtd, un_j, acc_j, u, an_j);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        an[c][j] = an_j[c];
-        us[c][j] = mk(u[0][c], u[1][c]);
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (a.stage < 4) {
-        xr_store_pairs<G>(row(a.out, c), tid, an[c]);
-        xr_store_pairs<G>(row(a.us, c), tid, us[c]);
-      } else {
-        xr_store_pairs<G>(row(a.out, c), tid, us[c]);
-      }
-    }
-    if (a.stage == 4 && a.cfl_rows != nullptr) {
-      T u[PJ][2][NC];
-#pragma unroll
-      for (int j = 0; j < PJ; ++j)
+        cx<T> un_j[NC], acc_j[NC], an_j[NC];
+        T u[2][NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          u[j][0][c] = us[c][j].x;
-          u[j][1][c] = us[c][j].y;
+          un_j[c] = un[c][j];
+          acc_j[c] = acc[c][j];
         }
-      cfl_store(u);
+        rk_pair<T, NC, MODE == XI_STAGE2>(a, v[j], dtd, un_j, acc_j, u, an_j);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          an[c][j] = an_j[c];
+          us[c][j] = mk(u[0][c], u[1][c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        if (a.stage < 4) {
+          xr_store_pairs<G>(row(a.out, c), tid, an[c]);
+          xr_store_pairs<G>(row(a.us, c), tid, us[c]);
+        } else {
+          xr_store_pairs<G>(row(a.out, c), tid, us[c]);
+        }
+      }
+      if (a.stage == 4 && a.cfl_rows != nullptr) {
+        T u[PJ][2][NC];
+#pragma unroll
+        for (int j = 0; j < PJ; ++j)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            u[j][0][c] = us[c][j].x;
+            u[j][1][c] = us[c][j].y;
+          }
+        cfl_store(u);
+      }
     }
   }
 }
@@ -1225,105 +1242,111 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (gated_off(a.active, a.act_all, slab)) return;
-  const int y = blockIdx.x;
-  XRSpec<T, NX, DA, NC> sp;
-  sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, NC, tid);
-  PH_XTW_FILL(a.tw, NX, NT, tid);
-  const XFwdArgs<T>& fa = a.stage < 4 ? m.fwd : m.fwd_last;
-  xr_tables<T, NX, DA>(fa.sig, fa.kk, s_sig, s_k, tid);
-  const long long base = (long long)slab * a.ss + (long long)y * NX;
-  // the RK combine's inputs (x pairs), per field: U (PH_XMID_PREFETCH 2), or
-  // U and the accumulator (1), loaded before the inverse transform so their
-  // latency overlaps it; the rest after it
-  constexpr int PF = NC == 8 ? PH_XMID_PREFETCH_MHD : PH_XMID_PREFETCH;
-  constexpr bool PRE_UN = PF != 0, PRE_ACC = PF == 1;
-  auto row = [&](T* f, int c) { return f + c * a.fs + base; };
-  cx<T> un[NC][PJ], acc[NC][PJ];
+  // PH_X_ROWS consecutive rows per block; the twiddle and mode tables are
+  // filled once
+#pragma unroll 1
+  for (int rr = 0; rr < PH_X_ROWS; ++rr) {
+    const int y = blockIdx.x * PH_X_ROWS + rr;
+    if (rr > 0) lds_barrier();  // the previous row is done with buf
+    XRSpec<T, NX, DA, NC> sp;
+    sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, NC, tid);
+    if (rr == 0) PH_XTW_FILL(a.tw, NX, NT, tid);
+    const XFwdArgs<T>& fa = a.stage < 4 ? m.fwd : m.fwd_last;
+    if (rr == 0) xr_tables<T, NX, DA>(fa.sig, fa.kk, s_sig, s_k, tid);
+    const long long base = (long long)slab * a.ss + (long long)y * NX;
+    // the RK combine's inputs (x pairs), per field: U (PH_XMID_PREFETCH 2), or
+    // U and the accumulator (1), loaded before the inverse transform so their
+    // latency overlaps it; the rest after it
+    constexpr int PF = NC == 8 ? PH_XMID_PREFETCH_MHD : PH_XMID_PREFETCH;
+    constexpr bool PRE_UN = PF != 0, PRE_ACC = PF == 1;
+    auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+    cx<T> un[NC][PJ], acc[NC][PJ];
 #pragma unroll
-  for (int c = 0; c < NC; ++c)
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        un[c][j] = mk(T(0), T(0));
+        acc[c][j] = mk(T(0), T(0));
+      }
+    if constexpr (PRE_UN) {
+      if (a.stage < 4) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+      }
+    }
+    if constexpr (PRE_ACC) {
+      if (a.stage > 1) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+      }
+    }
+    xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
+    const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
+    if constexpr (!PRE_UN) {
+      if (a.stage < 4) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
+      }
+    }
+    if constexpr (!PRE_ACC) {
+      if (a.stage > 1) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
+      }
+    }
+    T q[PJ][2][QN];
+    cx<T> an[NC][PJ];
 #pragma unroll
     for (int j = 0; j < PJ; ++j) {
-      un[c][j] = mk(T(0), T(0));
-      acc[c][j] = mk(T(0), T(0));
-    }
-  if constexpr (PRE_UN) {
-    if (a.stage < 4) {
+      const int p = G::point(tid, j);
+      T v[2][NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
-    }
-  }
-  if constexpr (PRE_ACC) {
-    if (a.stage > 1) {
+      for (int c = 0; c < NC; ++c) {
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        v[0][c] = z.x;
+        v[1][c] = z.y;
+      }
+      T u[2][NC];
+      cx<T> un_j[NC], acc_j[NC], an_j[NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
-    }
-  }
-  xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
-  const double dtd = a.dt_dev != nullptr ? a.dt_dev[a.dt_per_slab ? slab : 0] : a.dt;
-  if constexpr (!PRE_UN) {
-    if (a.stage < 4) {
+      for (int c = 0; c < NC; ++c) {
+        un_j[c] = un[c][j];
+        acc_j[c] = acc[c][j];
+      }
+      rk_pair<T, NC, NC == 4 || NC == 8>(a, v, dtd, un_j, acc_j, u, an_j);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(const_cast<T*>(a.un), c), tid, un[c]);
-    }
-  }
-  if constexpr (!PRE_ACC) {
-    if (a.stage > 1) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
-    }
-  }
-  T q[PJ][2][QN];
-  cx<T> an[NC][PJ];
-#pragma unroll
-  for (int j = 0; j < PJ; ++j) {
-    const int p = G::point(tid, j);
-    T v[2][NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
-      v[0][c] = z.x;
-      v[1][c] = z.y;
-    }
-    T u[2][NC];
-    cx<T> un_j[NC], acc_j[NC], an_j[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      un_j[c] = un[c][j];
-      acc_j[c] = acc[c][j];
-    }
-    rk_pair<T, NC, NC == 4 || NC == 8>(a, v, dtd, un_j, acc_j, u, an_j);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) an[c][j] = an_j[c];
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-      for (int c = 0; c < QN; ++c) q[j][r][c] = c < NC ? u[r][c] : T(0);
-  }
-  // keep_state (stage 4 also writes the new state) is an option of the
-  // unclamped 3D Euler form only, where the stage-4 accumulator is the state
-  if (a.stage < 4 || (NC == 5 && a.keep_state)) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, an[c]);
-  }
-  lds_barrier();  // buf is reused by the forward transform
-  if (a.stage < 4) {
-    T qf[PJ][2][XFwdShape<FMODE>::NC];
-#pragma unroll
-    for (int j = 0; j < PJ; ++j)
+      for (int c = 0; c < NC; ++c) an[c][j] = an_j[c];
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
-    xr_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, qf);
-  } else {
-    xr_forward<T, NX, DA, LMODE>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
+        for (int c = 0; c < QN; ++c) q[j][r][c] = c < NC ? u[r][c] : T(0);
+    }
+    // keep_state (stage 4 also writes the new state) is an option of the
+    // unclamped 3D Euler form only, where the stage-4 accumulator is the state
+    if (a.stage < 4 || (NC == 5 && a.keep_state)) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, an[c]);
+    }
+    lds_barrier();  // buf is reused by the forward transform
+    if (a.stage < 4) {
+      T qf[PJ][2][XFwdShape<FMODE>::NC];
+#pragma unroll
+      for (int j = 0; j < PJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int c = 0; c < XFwdShape<FMODE>::NC; ++c) qf[j][r][c] = q[j][r][c];
+      xr_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, qf);
+    } else {
+      xr_forward<T, NX, DA, LMODE>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
+    }
   }
 }
 
 // launch shape of the x kernels: threads per block and rows per block
 template <typename T, int NX>
 struct XLaunch {
-  static constexpr int threads = XRThreads<T, NX>::value, rows = 1;
+  static constexpr int threads = XRThreads<T, NX>::value, rows = PH_X_ROWS;
 };
 
 // ----------------------------------------------------------------------------
