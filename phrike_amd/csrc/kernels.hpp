@@ -235,6 +235,8 @@ struct ColArgs {
   cx<T>* out;
   long long out_fs, out_bs, out_ls;
   int ncol;   // valid columns
+  int ncolw;  // columns a store may cover (the row pitch; tiles then write
+              // whole 128-byte lines, zeros past ncol); 0: ncol
   int nc;     // output fields
   Keep keep;  // retained modes along this axis
   int ikflag; // COPY modes: multiply by i*k
@@ -320,6 +322,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   const Keep kp = a.keep;
   const long long ls = a.in_ls, ols = a.out_ls;
   const int ncol = a.ncol;
+  const int ncolw = a.ncolw > 0 ? a.ncolw : ncol;
   const int nc = a.nc;
   cx<T> X[PER][R];
   cx<T> W[P0::PER][P0::R];
@@ -346,9 +349,10 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
     return [=](int l, int n) {
       if constexpr (MODE == CM_INV)
-        return (c0 + l < ncol && kp.kept(n, N)) ? src[(long long)kp.compact(n, N) * ls + l] : mk(T(0), T(0));
+        return (c0 + l < ncol && kp.kept(n, N)) ? ld_stream(src + (long long)kp.compact(n, N) * ls + l)
+                                                : mk(T(0), T(0));
       else
-        return (c0 + l < ncol) ? src[(long long)n * ls + l] : mk(T(0), T(0));
+        return (c0 + l < ncol) ? ld_stream(src + (long long)n * ls + l) : mk(T(0), T(0));
     };
   };
   // visit every (q, r) output slot of this thread: f(q, r, l, n, col_ok)
@@ -360,7 +364,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       for (int r = 0; r < R; ++r) {
         int l, n;
         last_pos<N, TM, NT>(tid, q, r, l, n);
-        f(q, r, l, n, c0 + l < ncol);
+        f(q, r, l, n, c0 + l < ncolw);
       }
     }
   };
@@ -387,7 +391,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     if constexpr (MODE == CM_INV) {
       fwd(INV{});
       each([&](int q, int r, int l, int n, bool ok) {
-        if (ok) oc[(long long)n * ols + l] = X[q][r];
+        if (ok) st_stream(oc + (long long)n * ols + l, X[q][r]);
       });
     } else if constexpr (MODE == CM_FWD_COPY) {
       fwd(FWD{});
@@ -396,7 +400,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           const int j = kp.compact(n, N);
           cx<T> v = cscale(X[q][r], a.sig[j]);
           if (a.ikflag) v = mul_ik(v, a.kk[j]);
-          oc[(long long)j * ols + l] = v;
+          st_stream(oc + (long long)j * ols + l, v);
         }
       });
     } else if constexpr (MODE == CM_FWD_DIV3) {
@@ -407,7 +411,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          oc[(long long)j * ols + l] = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
+          st_stream(oc + (long long)j * ols + l, cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
         }
       });
       cx<T>* o2 = out + (long long)(nc + c) * a.out_fs + c0;
@@ -415,7 +419,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          o2[(long long)j * ols + l] = cscale(X[q][r], a.sig[j]);
+          st_stream(o2 + (long long)j * ols + l, cscale(X[q][r], a.sig[j]));
         }
       });
     } else {
@@ -447,7 +451,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       lds_barrier();
       fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
       each([&](int q, int r, int l, int n, bool ok) {
-        if (ok) oc[(long long)n * ols + l] = X[q][r];
+        if (ok) st_stream(oc + (long long)n * ols + l, X[q][r]);
       });
     }
   }
@@ -677,6 +681,11 @@ struct XRGeo {
   static constexpr int MX = DA ? NX / 3 + 1 : NX / 2 + 1;  // kept x modes
   static constexpr int K = ColElem<T>::PW;                 // modes per 16-byte chunk
   static constexpr int MC = (MX + K - 1) / K;              // chunks holding kept modes
+  // chunks per row pitch (mxp: 128-byte lines, 8 chunks).  Stores cover the
+  // whole pitch, zeros past the kept modes: a partly written cache line
+  // costs more than a whole one (15 rows of 171 FP64 chunks 2.73 ms, of 176
+  // chunks 2.01 ms at 512^3)
+  static constexpr int MCP = (MC + 7) / 8 * 8;
   static constexpr bool PALL = M >= NT;                    // every thread owns points
   static constexpr int PJ = PALL ? M / NT : 1;             // packed points per thread
   // FP32 threads own two adjacent points (x = 4t..4t+3, one 16-byte access);
@@ -690,14 +699,14 @@ struct XRGeo {
 template <typename G, typename T>
 __device__ __forceinline__ void xr_load_pairs(const T* row, int tid, cx<T> (&v)[G::PJ]) {
   if constexpr (G::CONSEC) {
-    const pair4f q = reinterpret_cast<const pair4f*>(row)[tid];
+    const pair4f q = ld_stream(reinterpret_cast<const pair4f*>(row) + tid);
     v[0] = mk(q.a, q.b);
     v[1] = mk(q.c, q.d);
   } else {
 #pragma unroll
     for (int j = 0; j < G::PJ; ++j) {
       const int p = G::point(tid, j);
-      v[j] = (G::PALL || p < G::M) ? reinterpret_cast<const cx<T>*>(row)[p] : mk(T(0), T(0));
+      v[j] = (G::PALL || p < G::M) ? ld_stream(reinterpret_cast<const cx<T>*>(row) + p) : mk(T(0), T(0));
     }
   }
 }
@@ -709,12 +718,12 @@ __device__ __forceinline__ void xr_store_pairs(T* row, int tid, const cx<T> (&v)
     q.b = v[0].y;
     q.c = v[1].x;
     q.d = v[1].y;
-    reinterpret_cast<pair4f*>(row)[tid] = q;
+    st_stream(reinterpret_cast<pair4f*>(row) + tid, q);
   } else {
 #pragma unroll
     for (int j = 0; j < G::PJ; ++j) {
       const int p = G::point(tid, j);
-      if (G::PALL || p < G::M) reinterpret_cast<cx<T>*>(row)[p] = v[j];
+      if (G::PALL || p < G::M) st_stream(reinterpret_cast<cx<T>*>(row) + p, v[j]);
     }
   }
 }
@@ -751,7 +760,7 @@ struct XRSpec {
       a[q] = C{};
       if (NTASK % G::NT == 0 || t < NTASK) {
         const int l = t / G::MC, m = t - l * G::MC;
-        if (l < nfl) a[q] = s[l * fsc + m];
+        if (l < nfl) a[q] = ld_stream(s + l * fsc + m);
       }
     }
   }
@@ -811,11 +820,11 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
                                                const T* sig, const T* kk, bool ik, int tid) {
   using G = XRGeo<T, NX, DA>;
   using C = typename ChunkT<T, G::K>::type;
-  constexpr int NTASK = LX * G::MC, M = G::M;
+  constexpr int NTASK = LX * G::MCP, M = G::M;
   C* d = reinterpret_cast<C*>(dst);
   const long long fsc = fs / G::K;
   for (int t = tid; t < NTASK; t += G::NT) {
-    const int l = t / G::MC, m = t - l * G::MC;
+    const int l = t / G::MCP, m = t - l * G::MCP;
     if (l < nl) {
       C c{};
 #pragma unroll
@@ -832,7 +841,7 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
           ch_set(c, i, X);
         }
       }
-      d[l * fsc + m] = c;
+      st_stream(d + l * fsc + m, c);
     }
   }
 }

@@ -150,11 +150,23 @@ inline bool supported_len(int n) { return n >= 8 && n <= 1024 && is_pow2(n); }
 
 // the x kernels are instantiated for both kept-mode counts: N/3 + 1 (2/3 rule)
 // and N/2 + 1
+// the x-forward stores cover the row pitch the kernel assumes (XRGeo::MCP chunks)
+template <typename T, int N, bool DA>
+void check_x_pitch(int mxp) {
+  using G = XRGeo<T, N, DA>;
+  if (mxp != G::MCP * G::K) throw Error(-9, "spectral row pitch does not match the x kernels");
+}
 template <typename T, int N, int MODE>
 void launch_xfwd_n(int mx, dim3 grid, const XFwdArgs<T>& a, hipStream_t st) {
-  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xfwd<T, N, true, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
-  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xfwd<T, N, false, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
-  else throw Error(-2, "unsupported x mode count");
+  if (mx == N / 3 + 1) {
+    check_x_pitch<T, N, true>(a.mxp);
+    hipLaunchKernelGGL((k_xfwd<T, N, true, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else if (mx == N / 2 + 1) {
+    check_x_pitch<T, N, false>(a.mxp);
+    hipLaunchKernelGGL((k_xfwd<T, N, false, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else {
+    throw Error(-2, "unsupported x mode count");
+  }
 }
 template <typename T, int MODE>
 void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t st) {
@@ -183,9 +195,17 @@ void launch_xinv(int nx, int nrows, int nslab, const XInvArgs<T>& a, hipStream_t
 
 template <typename T, int N, int NC, int FMODE>
 void launch_xmid_n(int mx, dim3 grid, const XMidArgs<T>& a, hipStream_t st) {
-  if (mx == N / 3 + 1) hipLaunchKernelGGL((k_xmid<T, N, true, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
-  else if (mx == N / 2 + 1) hipLaunchKernelGGL((k_xmid<T, N, false, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
-  else throw Error(-2, "unsupported x mode count");
+  if (mx == N / 3 + 1) {
+    check_x_pitch<T, N, true>(a.fwd.mxp);
+    check_x_pitch<T, N, true>(a.fwd_last.mxp);
+    hipLaunchKernelGGL((k_xmid<T, N, true, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else if (mx == N / 2 + 1) {
+    check_x_pitch<T, N, false>(a.fwd.mxp);
+    check_x_pitch<T, N, false>(a.fwd_last.mxp);
+    hipLaunchKernelGGL((k_xmid<T, N, false, NC, FMODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else {
+    throw Error(-2, "unsupported x mode count");
+  }
 }
 template <typename T, int NC, int FMODE>
 void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
@@ -310,8 +330,10 @@ struct Plan3D : PlanBase {
     dy.upload(ay, 1.0);
     dz.upload(az, 1.0);
     {
-      const int ncol = (mx + PW - 1) / PW;
-      std::vector<CE> h(ncol);
+      // by column of the row pitch (the column passes visit whole tiles),
+      // zero past the kept modes
+      const int ncol = (mx + PW - 1) / PW, ncolw = mxp / PW;
+      std::vector<CE> h(ncolw, CE(0.0));
       for (int c = 0; c < ncol; ++c) {
         if constexpr (PW == 2) {
           const double v0 = ax.sm1[2 * c], v1 = 2 * c + 1 < mx ? ax.sm1[2 * c + 1] : 0.0;
@@ -320,8 +342,8 @@ struct Plan3D : PlanBase {
           h[c] = CE(ax.sm1[c]);
         }
       }
-      hip_check(hipMalloc(&xcol_sm1, sizeof(CE) * ncol), "hipMalloc(column sigma-1)");
-      hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncol, hipMemcpyHostToDevice), "copy column sigma-1");
+      hip_check(hipMalloc(&xcol_sm1, sizeof(CE) * ncolw), "hipMalloc(column sigma-1)");
+      hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column sigma-1");
     }
     {
       std::vector<T> ones(mx, T(1));
@@ -424,6 +446,7 @@ struct Plan3D : PlanBase {
     a.out_bs = b_zs() / PW;
     a.out_ls = b_ys() / PW;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = nc;
     a.keep = ay.keep();
     a.ikflag = ikflag;
@@ -449,6 +472,7 @@ struct Plan3D : PlanBase {
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = nc;
     a.keep = az.keep();
     a.ikflag = ikflag;
@@ -474,6 +498,7 @@ struct Plan3D : PlanBase {
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = nc;
     a.keep = az.keep();
     a.tw = dz.tw;
@@ -499,6 +524,7 @@ struct Plan3D : PlanBase {
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = nc;
     a.keep = az.keep();
     a.tw = dz.tw;
@@ -525,6 +551,7 @@ struct Plan3D : PlanBase {
     a.out_bs = ((long long)ny * mxp) / PW;
     a.out_ls = (mxp) / PW;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = nc;
     a.keep = ay.keep();
     a.ikflag = 0;
@@ -828,6 +855,7 @@ struct Plan2D : PlanBase {
     a.out_bs = a.in_bs;
     a.out_ls = a.in_ls;
     a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
     a.nc = 4;
     a.keep = ay.keep();
     a.ikflag = ikflag;
