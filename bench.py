@@ -706,6 +706,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-run-loop", action="store_true",
+                    help="skip the device run-loop timing (profiling passes of the step alone)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the FP32-vs-FP64 turbulence study")
     ap.add_argument("--precision-size", type=int, default=512)
@@ -737,8 +739,8 @@ def main():
         tab = traffic_table(*args.traffic, args.size, args.precision,
                             command="rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE -f csv -- python3 bench.py "
                                     f"--size {args.size} --precision {args.precision} --steps 2 --warmup 0 "
-                                    "--no-cpu --no-extras --no-kernel-timing; and -- python3 bench.py "
-                                    "--calibrate-copy")
+                                    "--no-cpu --no-extras --no-kernel-timing --no-run-loop; and -- python3 "
+                                    "bench.py --calibrate-copy")
         out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
         old = {}
         if os.path.exists(out):
@@ -858,9 +860,10 @@ def main():
     sums = s.conserved_sums(U)
     del out
     extras = {}
-    print("[bench] device run loop (dt from CFL every step)", file=sys.stderr, flush=True)
-    extras["run_loop"] = run_loop_timing(s, U, args.steps)
-    extras["run_loop"]["overhead_vs_fixed_dt_ms"] = round(extras["run_loop"]["ms_per_step"] - ms_per_step, 3)
+    if not args.no_run_loop:
+        print("[bench] device run loop (dt from CFL every step)", file=sys.stderr, flush=True)
+        extras["run_loop"] = run_loop_timing(s, U, args.steps)
+        extras["run_loop"]["overhead_vs_fixed_dt_ms"] = round(extras["run_loop"]["ms_per_step"] - ms_per_step, 3)
     if not args.no_extras:
         extras["hbm_reference"] = hbm_reference(dev)
         if roof is not None:
