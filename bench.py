@@ -48,6 +48,9 @@ def byte_model(n: int):
         # unit-factor y pass, (sx sy sz - 1) z pass, x inverse added to it
         "xmid_last_keep": 5 * a + 5 + 5 * a + 5, "yfwd_unit": 5 * a + 5 * b,
         "z_corr": 10 * b, "xinv_add": 5 * a + 5 + 5,
+        # chained steps: the filter's x inverse also forms the next step's
+        # fluxes and their x transform (the state is not read again)
+        "xend_out": 5 * a + 5 + 15 * a, "xend_add": 5 * a + 5 + 5 + 15 * a,
     }
     # per-step model of the unfused pencil pipeline (stage states written and
     # re-read), the official figure for hbm_frac_step
@@ -68,15 +71,17 @@ def kernel_symbol(tag: str, n: int, prec: str) -> str:
             "xmid1": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid23": f"k_xmid<{T}, {n}, true, 5, 0>",
             "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid_last_keep": f"k_xmid<{T}, {n}, true, 5, 0>",
             "yfwd_unit": f"k_col<{C}, {n}, 1>", "z_corr": f"k_col<{C}, {n}, 5>",
-            "xinv_add": f"k_xinv<{T}, {n}, true, 5, 4>"}.get(tag, tag)
+            "xinv_add": f"k_xinv<{T}, {n}, true, 5, 4>",
+            "xend_out": f"k_xend<{T}, {n}, true, 0>", "xend_add": f"k_xend<{T}, {n}, true, 4>"}.get(tag, tag)
 
 
-# launches of each timing tag in one RK4 step of the fused 3D pipeline
-STEP_LAUNCHES = {"xfwd_flux": 1, "yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
-                 "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xinv_out": 1}
+# launches of each timing tag in one RK4 step of the fused 3D pipeline, in
+# a chain of steps (each step's last kernel forms the next step's first x pass)
+STEP_LAUNCHES = {"yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
+                 "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xend_out": 1}
 # FP32 steps on a previous step's output (the filter in correction form)
-STEP_LAUNCHES_CORR = {"xfwd_flux": 1, "yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
-                      "xmid_last_keep": 1, "yfwd_unit": 1, "z_corr": 1, "xinv_add": 1}
+STEP_LAUNCHES_CORR = {"yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
+                      "xmid_last_keep": 1, "yfwd_unit": 1, "z_corr": 1, "xend_add": 1}
 
 
 def step_launches(prec: str):

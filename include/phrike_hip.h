@@ -101,6 +101,17 @@ int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const doubl
  * the flag (direct form, the reference's operation order).  No reference
  * counterpart: a precision choice of this backend's FP32 plans. */
 #define PH_STEP_BANDLIMITED 1
+/* PH_STEP_CHAIN: end the step with the x transform of its output's fluxes
+ * (the next step's first pass, formed while the output is in registers) and
+ * keep them in the workspace.  PH_STEP_CHAINED: U is the unmodified output of
+ * a PH_STEP_CHAIN step of this plan; if the workspace still holds U's fluxes
+ * the step skips its own first pass.  That is checked by address (the plan,
+ * the workspace, U equal to that step's output, and no call since that wrote
+ * the workspace's spectral buffers); otherwise the step computes them.  The
+ * caller promises only that U's contents are unchanged.  No reference
+ * counterpart: a fusion across the reference's step boundary. */
+#define PH_STEP_CHAIN 4
+#define PH_STEP_CHAINED 8
 int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, int flags,
                  void* ws, void* stream);
 

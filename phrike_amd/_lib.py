@@ -17,6 +17,8 @@ PH_F64 = 0
 PH_F32 = 1
 PH_STEP_BANDLIMITED = 1
 PH_STEP_SOLENOIDAL = 2
+PH_STEP_CHAIN = 4
+PH_STEP_CHAINED = 8
 # device-resident run loop scalars (ph_run3d / ph_run2d)
 PH_RS_T, PH_RS_DT, PH_RS_TEND, PH_RS_TSTOP, PH_RS_DTNEXT, PH_RS_DTUSED, PH_RS_N = range(7)
 PH_RF_ACTIVE, PH_RF_STEPS, PH_RF_N = range(3)

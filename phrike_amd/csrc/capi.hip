@@ -160,6 +160,7 @@ int ph_plan3d_workspace_bytes(ph_plan3d_t plan, size_t* bytes) {
 int ph_rhs3d(ph_plan3d_t plan, const void* U, void* rhs, void* ws, void* stream) {
   return guard([&] {
     need(U && rhs && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
       p.rhs(static_cast<const R*>(U), static_cast<R*>(rhs), ws, S(stream));
@@ -171,6 +172,7 @@ int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out
                void* stream) {
   return guard([&] {
     need(f && out && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     need(axis >= -1 && axis <= 2, "axis must be -1, 0, 1 or 2");
     need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
     with3(plan, [&](auto& p) {
@@ -184,13 +186,20 @@ int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const do
                  void* ws, void* stream) {
   return guard([&] {
     need(U && out && ws, "null buffer");
-    need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
+    need((flags & ~(PH_STEP_BANDLIMITED | PH_STEP_CHAIN | PH_STEP_CHAINED)) == 0, "unknown step flags");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
       const size_t n = 5ull * size_t(p.ncell) * sizeof(R);
       need(!ph::ranges_overlap(U, n, out, n), "U and out must not overlap");
+      auto& reg = ph::chain_registry();
+      const bool chain = (flags & PH_STEP_CHAIN) != 0;
+      // the fluxes of U are in the workspace only if this plan's previous
+      // chained step wrote U there and nothing has written the workspace since
+      const bool flux_in = (flags & PH_STEP_CHAINED) != 0 && reg.holds(ws, &p, U);
+      reg.forget(ws);
       p.step(static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, ws, S(stream),
-             (flags & PH_STEP_BANDLIMITED) != 0);
+             (flags & PH_STEP_BANDLIMITED) != 0, nullptr, false, flux_in, chain);
+      if (chain) reg.set(ws, &p, out);
     });
   });
 }
@@ -206,6 +215,7 @@ int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, i
     need(Ua && Ub && rs_dev && rf_dev && ws, "null buffer");
     need(nsteps >= 1, "nsteps must be >= 1");
     need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
+    ph::chain_registry().forget(ws);
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
       const size_t n = 5ull * size_t(p.ncell) * sizeof(R);
@@ -219,8 +229,11 @@ int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, i
         // steps after the first take a previous step's (band-limited) output
         // the step's last kernel leaves the new state's per-row wave-speed
         // maxima behind, so the next dt needs no second read of the state
+        // steps chain within the call: each but the last leaves the fluxes of
+        // its output for the next (a step of a stopped run does nothing, and
+        // so do all after it)
         p.step(src, dst, 0.0, rs_dev + ph::RS_DT, ws, st, k > 0 || (flags & PH_STEP_BANDLIMITED) != 0, rf_dev,
-               true);
+               true, k > 0, k + 1 < nsteps);
         p.max_rows(nullptr, rs_dev + ph::RS_DTNEXT, cfl, ws, st, rf_dev);
         hipLaunchKernelGGL(ph::k_run_advance, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, rf_dev);
         ph::check_launch("k_run_advance");
@@ -253,6 +266,7 @@ int ph_conserved_sums3d(ph_plan3d_t plan, const void* U, double* out5_dev, void*
 int ph_fft3d(ph_plan3d_t plan, int nfields, const void* f, void* spec, void* ws, void* stream) {
   return guard([&] {
     need(f && spec && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
@@ -264,6 +278,7 @@ int ph_fft3d(ph_plan3d_t plan, int nfields, const void* f, void* spec, void* ws,
 int ph_ifft3d(ph_plan3d_t plan, int nfields, const void* spec, void* f, void* ws, void* stream) {
   return guard([&] {
     need(f && spec && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     need(nfields >= 1 && nfields <= 5, "nfields must be in [1, 5]");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
@@ -318,6 +333,7 @@ int ph_plan2d_workspace_bytes(ph_plan2d_t plan, size_t* bytes) {
 int ph_rhs2d(ph_plan2d_t plan, int nbatch, const void* U, void* rhs, void* ws, void* stream) {
   return guard([&] {
     need(U && rhs && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     with2(plan, [&](auto& p) {
       PH_R2;
       p.check_nb(nbatch);
@@ -330,6 +346,7 @@ int ph_deriv2d(ph_plan2d_t plan, int nbatch, int axis, const void* f, void* out,
                void* stream) {
   return guard([&] {
     need(f && out && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     need(axis >= -1 && axis <= 1, "axis must be -1, 0 or 1");
     with2(plan, [&](auto& p) {
       PH_R2;
@@ -343,6 +360,7 @@ int ph_step2d(ph_plan2d_t plan, int nbatch, const void* U, void* out, double dt,
               const int* active_dev, int apply_filter, void* ws, void* stream) {
   return guard([&] {
     need(U && out && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     with2(plan, [&](auto& p) {
       PH_R2;
       p.check_nb(nbatch);
@@ -391,6 +409,7 @@ int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, doubl
              void* ws, void* stream) {
   return guard([&] {
     need(Ua && Ub && rs_dev && rf_dev && ws, "null buffer");
+    ph::chain_registry().forget(ws);
     need(nsteps >= 1, "nsteps must be >= 1");
     with2(plan, [&](auto& p) {
       PH_R2;

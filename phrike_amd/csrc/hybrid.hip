@@ -490,6 +490,7 @@ int ph_rhs2dh(ph_planh_t plan, int nbatch, const void* U, void* rhs, void* ws, v
     withh(plan, [&](auto& p) {
       PH_RH;
       if (nbatch < 1 || nbatch > p.batch || !U || !rhs || !ws) throw ph::Error(PH_ERR_ARG, "bad arguments");
+      ph::chain_registry().forget(ws);
       p.rhs(nbatch, static_cast<const R*>(U), static_cast<R*>(rhs), nullptr, ws,
             reinterpret_cast<hipStream_t>(stream));
     });
@@ -502,6 +503,7 @@ int ph_step2dh(ph_planh_t plan, int nbatch, const void* U, void* out, double dt,
     withh(plan, [&](auto& p) {
       PH_RH;
       if (nbatch < 1 || nbatch > p.batch || !U || !out || !ws) throw ph::Error(PH_ERR_ARG, "bad arguments");
+      ph::chain_registry().forget(ws);
       const size_t n = 4ull * size_t(nbatch) * size_t(p.ncell) * sizeof(R);
       if (ph::ranges_overlap(U, n, out, n)) throw ph::Error(PH_ERR_ARG, "U and out must not overlap");
       p.step(nbatch, static_cast<const R*>(U), static_cast<R*>(out), dt, dt_dev, active_dev, apply_filter != 0,

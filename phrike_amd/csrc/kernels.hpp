@@ -669,9 +669,19 @@ struct XMidArgs {
 // FP64, NX/4 FP32 threads), so several share a CU and the load, transform
 // and store phases of different blocks overlap.
 // ----------------------------------------------------------------------------
+// NX / PH_XR_DIV threads per row (at least one wave): FP64 threads own
+// 2/PH_XR_DIV_F64 * ... packed points, FP32 threads two adjacent pairs at the
+// default
+#ifndef PH_XR_DIV_F64
+#define PH_XR_DIV_F64 2
+#endif
+#ifndef PH_XR_DIV_F32
+#define PH_XR_DIV_F32 4
+#endif
 template <typename T, int NX>
 struct XRThreads {
-  static constexpr int value = sizeof(T) == 8 ? clampi(NX / 2, 64, 512) : clampi(NX / 4, 64, 256);
+  static constexpr int value =
+      sizeof(T) == 8 ? clampi(NX / PH_XR_DIV_F64, 64, 512) : clampi(NX / PH_XR_DIV_F32, 64, 256);
 };
 
 template <typename T, int NX, bool DA>
@@ -1349,6 +1359,72 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
     } else {
       xr_forward<T, NX, DA, LMODE>(m.fwd_last, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
     }
+  }
+}
+
+// Fused step boundary between two steps: the post-step filter's x inverse
+// (XI_OUT: the filtered state; XI_ADD: the state plus the FP32 filter
+// correction) is written as the new state, with its per-row CFL maxima, and
+// then the fluxes of that state and their x transform (k_xfwd's work for the
+// next step's first stage) go to the spectral buffer, so the next step does
+// not read the state again.  The block loads its spectral row completely
+// before it writes that row.
+template <typename T, int NX, bool DA, int MODE>
+__global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xend(XMidArgs<T> m) {
+  using G = XRGeo<T, NX, DA>;
+  constexpr int NT = G::NT, PJ = G::PJ, NC = 5;
+  static_assert(MODE == XI_OUT || MODE == XI_ADD, "the filter's last pass writes or adds");
+  PH_XTW_DECL(T, NX);
+  __shared__ cx<T> buf[xrbuf_size<NX, NC>()];
+  __shared__ T s_sig[G::MX], s_k[G::MX];
+  const XInvArgs<T>& a = m.inv;
+  const int tid = threadIdx.x;
+  const int slab = blockIdx.y;
+  if (gated_off(a.active, a.act_all, slab)) return;
+#pragma unroll 1
+  for (int rr = 0; rr < PH_X_ROWS; ++rr) {
+    const int y = blockIdx.x * PH_X_ROWS + rr;
+    if (rr > 0) lds_barrier();  // the previous row is done with buf
+    XRSpec<T, NX, DA, NC> sp;
+    sp.load(a.in + (long long)slab * a.in_ss + (long long)y * a.mxp, a.in_fs, NC, tid);
+    if (rr == 0) PH_XTW_FILL(a.tw, NX, NT, tid);
+    if (rr == 0) xr_tables<T, NX, DA>(m.fwd.sig, m.fwd.kk, s_sig, s_k, tid);
+    const long long base = (long long)slab * a.ss + (long long)y * NX;
+    auto row = [&](T* f, int c) { return f + c * a.fs + base; };
+    cx<T> o[NC][PJ];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) o[c][j] = mk(T(0), T(0));
+      if constexpr (MODE == XI_ADD) xr_load_pairs<G>(row(a.out, c), tid, o[c]);
+    }
+    xr_inverse<T, NX, DA, NC>(sp, buf, tw, tid);
+    T q[PJ][2][NC];
+#pragma unroll
+    for (int j = 0; j < PJ; ++j) {
+      const int p = G::point(tid, j);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        o[c][j] = MODE == XI_ADD ? mk(o[c][j].x + z.x, o[c][j].y + z.y) : z;
+        q[j][0][c] = o[c][j].x;
+        q[j][1][c] = o[c][j].y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, o[c]);
+    if (a.cfl_rows != nullptr) {
+      double mm = 0.0;
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        if (!G::PALL && G::point(tid, j) >= G::M) continue;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) mm = dmax_nan(mm, wave_speed_v<T, 3>(q[j][r], a.gamma));
+      }
+      block_max_store<NT>(mm, a.cfl_rows + (long long)slab * (gridDim.x * PH_X_ROWS) + y, tid);
+    }
+    lds_barrier();  // buf is reused by the forward transform
+    xr_forward<T, NX, DA, XF_FLUX3>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
   }
 }
 

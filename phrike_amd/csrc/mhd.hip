@@ -439,6 +439,7 @@ int ph_planmhd3d_workspace_bytes(ph_planmhd3d_t plan, size_t* bytes) {
 int ph_rhs_mhd3d(ph_planmhd3d_t plan, const void* U, void* rhs, void* ws, void* stream) {
   return mguard([&] {
     if (!U || !rhs || !ws) throw ph::Error(PH_ERR_ARG, "null buffer");
+    ph::chain_registry().forget(ws);
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;
       p.rhs(static_cast<const R*>(U), static_cast<R*>(rhs), ws, MS(stream));
@@ -449,6 +450,7 @@ int ph_rhs_mhd3d(ph_planmhd3d_t plan, const void* U, void* rhs, void* ws, void* 
 int ph_project_mhd3d(ph_planmhd3d_t plan, void* U, void* ws, void* stream) {
   return mguard([&] {
     if (!U || !ws) throw ph::Error(PH_ERR_ARG, "null buffer");
+    ph::chain_registry().forget(ws);
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;
       p.project(static_cast<R*>(U), ws, MS(stream));
@@ -460,6 +462,7 @@ int ph_step_mhd3d_ex(ph_planmhd3d_t plan, const void* U, void* out, double dt, i
                      void* stream) {
   return mguard([&] {
     if (!U || !out || !ws) throw ph::Error(PH_ERR_ARG, "null buffer");
+    ph::chain_registry().forget(ws);
     if (flags & ~PH_STEP_SOLENOIDAL) throw ph::Error(PH_ERR_ARG, "unknown step flags");
     withm(plan, [&](auto& p) {
       using R = Elem<decltype(p)>;

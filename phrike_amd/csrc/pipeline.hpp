@@ -212,7 +212,52 @@ void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t
   PH_SWITCH_N(nx, (launch_xmid_n<T, N_, NC, FMODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
+template <typename T, int N, int MODE>
+void launch_xend_n(int mx, dim3 grid, const XMidArgs<T>& a, hipStream_t st) {
+  if (mx == N / 3 + 1) {
+    check_x_pitch<T, N, true>(a.fwd.mxp);
+    hipLaunchKernelGGL((k_xend<T, N, true, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else if (mx == N / 2 + 1) {
+    check_x_pitch<T, N, false>(a.fwd.mxp);
+    hipLaunchKernelGGL((k_xend<T, N, false, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+  } else {
+    throw Error(-2, "unsupported x mode count");
+  }
+}
+template <typename T, int MODE>
+void launch_xend(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
+  PH_SWITCH_N(nx, (launch_xend_n<T, N_, MODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
+}
+
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+// Chained steps.  A step asked to chain (PH_STEP_CHAIN) ends with k_xend,
+// which leaves the x-transformed fluxes of the new state in the workspace's
+// spectral buffer; the next chained step of the same plan on that state and
+// workspace then skips its first x pass.  What a workspace holds is recorded
+// here, by workspace address: a chained step records (plan, state), every
+// other operation that writes a workspace's spectral buffers forgets it.
+struct ChainRegistry {
+  std::mutex mu;
+  std::map<const void*, std::pair<const void*, const void*>> held;  // ws -> (plan, state)
+  void set(const void* ws, const void* plan, const void* state) {
+    std::lock_guard<std::mutex> g(mu);
+    held[ws] = {plan, state};
+  }
+  void forget(const void* ws) {
+    std::lock_guard<std::mutex> g(mu);
+    held.erase(ws);
+  }
+  bool holds(const void* ws, const void* plan, const void* state) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = held.find(ws);
+    return it != held.end() && it->second.first == plan && it->second.second == state;
+  }
+};
+inline ChainRegistry& chain_registry() {
+  static ChainRegistry r;
+  return r;
+}
 
 // D.U on matrix cores (gll.hip): out[b] = D F[b] for b < nbatch, or the
 // hybrid-RHS epilogue out = -(add + D F) when add is not null.  With `active`,
@@ -625,10 +670,14 @@ struct Plan3D : PlanBase {
   // cfl: also leave the new state's per-row wave-speed maxima in CflRows(ws)
   // (the last kernel takes them from the values it writes); max_rows() then
   // gives the max and dt without reading the state again
+  // flux_in: the spectral buffer already holds the x-transformed fluxes of U
+  // (left by the previous step's k_xend), so the first x pass is skipped;
+  // flux_out: end with k_xend, which leaves those of out for the next step.
   void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st,
-            bool bandlimited = false, const int* gate = nullptr, bool cfl = false) const {
+            bool bandlimited = false, const int* gate = nullptr, bool cfl = false, bool flux_in = false,
+            bool flux_out = false) const {
     const bool corr = bandlimited && sizeof(T) == 4;
-    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st, gate);
+    if (!flux_in) xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st, gate);
     for (int s = 1; s <= 4; ++s) {
       yfwd(true, 5, 0, ws, st, false, gate);
       zpass(true, 5, 0, ws, st, gate);
@@ -658,16 +707,25 @@ struct Plan3D : PlanBase {
       yfwd(false, 5, 0, ws, st, true, gate);
       zcorr(5, ws, st, gate);
       yinv(5, ws, st, gate);
-      KScope ks("xinv_add", st);
-      launch_xinv<T, 5, XI_ADD>(nx, ny, nz, a, st);
     } else {
       yfwd(false, 5, 0, ws, st, false, gate);
       zpass(false, 5, 0, ws, st, gate);
       yinv(5, ws, st, gate);
-      KScope ks("xinv_out", st);
-      launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
     }
-    check_launch("k_xinv");
+    if (flux_out) {
+      XMidArgs<T> m{};
+      m.inv = a;
+      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws, gate);
+      KScope ks(corr ? "xend_add" : "xend_out", st);
+      if (corr) launch_xend<T, XI_ADD>(nx, ny, nz, m, st);
+      else launch_xend<T, XI_OUT>(nx, ny, nz, m, st);
+      check_launch("k_xend");
+    } else {
+      KScope ks(corr ? "xinv_add" : "xinv_out", st);
+      if (corr) launch_xinv<T, 5, XI_ADD>(nx, ny, nz, a, st);
+      else launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+      check_launch("k_xinv");
+    }
   }
 
   // Unnormalised forward transform of nf real fields to their half spectra

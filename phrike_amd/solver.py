@@ -130,12 +130,17 @@ class SpectralSolver3D:
         When U is this solver's own previous output, unmodified since (same
         tensor object, same version counter), it has no content beyond the 2/3
         cutoff and the step says so (PH_STEP_BANDLIMITED): FP32 plans then
-        filter in correction form.  Any other input takes the direct form."""
+        filter in correction form.  Any other input takes the direct form.
+        Every step leaves the x-transformed fluxes of its output in the
+        workspace (PH_STEP_CHAIN); the next step on that unmodified output
+        says so (PH_STEP_CHAINED) and, if nothing has used the workspace in
+        between, skips its first x pass."""
         _check_pair(self._state(U, copy=False), out, "out")
         _check_scalars(dt_dev, 1, torch.float64, U.device, "dt_dev")
         plan, ws, st = self._ctx()
         last = self._last_out() if self._last_out is not None else None
-        flags = _lib.PH_STEP_BANDLIMITED if (last is U and U._version == self._last_version) else 0
+        own = last is U and U._version == self._last_version
+        flags = _lib.PH_STEP_CHAIN | ((_lib.PH_STEP_BANDLIMITED | _lib.PH_STEP_CHAINED) if own else 0)
         _lib.call("ph_step3d_ex", plan, _lib.ptr(U), _lib.ptr(out), float(dt), _lib.ptr(dt_dev), flags, ws, st)
         self._last_out = weakref.ref(out)
         self._last_version = out._version

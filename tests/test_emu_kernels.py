@@ -341,3 +341,33 @@ def test_fused_cfl_equals_standalone_max(emu):
     assert emu.ph_run2d(e.h, nb, P(U), P(V), 1, P(rs), P(rf), cfl, P(e.ws), None) == 0
     _, d1 = e.dt(V, cfl)
     assert rs[nb] == d1[0] and rs[nb + 2] == d1[2] and rs[nb + 1] == 0.125
+
+
+def test_chained_steps(emu):
+    """PH_STEP_CHAIN / PH_STEP_CHAINED: steps whose last kernel forms the next
+    step's fluxes (k_xend) give bit for bit the plain steps' states, FP64 and
+    FP32 (correction-form filter); a call that writes the workspace between
+    two chained steps makes the second compute its fluxes itself, and an
+    unknown flag is an argument error."""
+    from emu_lib import Emu3D
+    from phrike_amd import _lib
+    from phrike_amd.initial_conditions import smooth_random_state_3d
+    CH, CD, BL = _lib.PH_STEP_CHAIN, _lib.PH_STEP_CHAINED, _lib.PH_STEP_BANDLIMITED
+    U0 = smooth_random_state_3d(16, 16, 16, seed=5, kmax=4)
+    for dtype in (np.float64, np.float32):
+        e = Emu3D(emu, 16, 16, 16, dtype=dtype)
+        dt = 2e-3
+        A = U0.astype(dtype)
+        for k in range(3):
+            A = e.step(A, dt, BL if k else 0)
+        B = U0.astype(dtype)
+        for k in range(3):
+            B = e.step(B, dt, CH | ((BL | CD) if k else 0))
+        np.testing.assert_array_equal(A, B)
+        C = e.step(U0.astype(dtype), dt, CH)
+        e.deriv(0, C)  # writes the spectral buffers: the held fluxes are gone
+        C = e.step(C, dt, CH | CD | BL)
+        D = e.step(e.step(U0.astype(dtype), dt), dt, BL)
+        np.testing.assert_array_equal(C, D)
+    with pytest.raises(RuntimeError, match="flags"):
+        e.step(U0.astype(np.float32), dt, 16)
