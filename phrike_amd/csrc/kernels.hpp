@@ -669,9 +669,9 @@ struct XMidArgs {
 // FP64, NX/4 FP32 threads), so several share a CU and the load, transform
 // and store phases of different blocks overlap.
 // ----------------------------------------------------------------------------
-// NX / PH_XR_DIV threads per row (at least one wave): FP64 threads own
-// 2/PH_XR_DIV_F64 * ... packed points, FP32 threads two adjacent pairs at the
-// default
+// threads per row: NX / PH_XR_DIV_F64 (FP64) or NX / PH_XR_DIV_F32 (FP32), at
+// least one wave; a thread owns (NX/2) / threads packed points (at the
+// defaults one in FP64, two adjacent ones in FP32: one 16-byte access)
 #ifndef PH_XR_DIV_F64
 #define PH_XR_DIV_F64 2
 #endif
