@@ -424,7 +424,10 @@ int ph_run2d(ph_plan2d_t plan, int nbatch, void* Ua, void* Ub, int nsteps, doubl
         R* dst = static_cast<R*>(k % 2 ? Ua : Ub);
         hipLaunchKernelGGL(ph::k_run_clip, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, (const int*)rf_dev);
         ph::check_launch("k_run_clip");
-        p.step(nbatch, src, dst, 0.0, rs_dev + ph::RS_DT * nbatch, act, true, ws, st, true);
+        // steps chain within the call (each but the last forms the next
+        // step's fluxes; runs that stop stay stopped for the rest of the call)
+        p.step(nbatch, src, dst, 0.0, rs_dev + ph::RS_DT * nbatch, act, true, ws, st, true, k > 0,
+               k + 1 < nsteps);
         p.max_rows(nbatch, nullptr, rs_dev + ph::RS_DTNEXT * nbatch, cfl, ws, st, act);
         hipLaunchKernelGGL(ph::k_run_advance, dim3(nblk), dim3(ph::NT), 0, st, nbatch, rs_dev, rf_dev);
         ph::check_launch("k_run_advance");

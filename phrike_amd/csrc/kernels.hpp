@@ -1369,10 +1369,12 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
 // next step's first stage) go to the spectral buffer, so the next step does
 // not read the state again.  The block loads its spectral row completely
 // before it writes that row.
-template <typename T, int NX, bool DA, int MODE>
+template <typename T, int NX, bool DA, int NC, int MODE>
 __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void k_xend(XMidArgs<T> m) {
   using G = XRGeo<T, NX, DA>;
-  constexpr int NT = G::NT, PJ = G::PJ, NC = 5;
+  constexpr int NT = G::NT, PJ = G::PJ;
+  constexpr int FMODE = NC == 5 ? XF_FLUX3 : XF_FLUX2;  // 3D or batched-2D Euler state
+  static_assert(NC == 4 || NC == 5, "Euler states");
   static_assert(MODE == XI_OUT || MODE == XI_ADD, "the filter's last pass writes or adds");
   PH_XTW_DECL(T, NX);
   __shared__ cx<T> buf[xrbuf_size<NX, NC>()];
@@ -1419,12 +1421,12 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       for (int j = 0; j < PJ; ++j) {
         if (!G::PALL && G::point(tid, j) >= G::M) continue;
 #pragma unroll
-        for (int r = 0; r < 2; ++r) mm = dmax_nan(mm, wave_speed_v<T, 3>(q[j][r], a.gamma));
+        for (int r = 0; r < 2; ++r) mm = dmax_nan(mm, wave_speed_v<T, NC - 2>(q[j][r], a.gamma));
       }
       block_max_store<NT>(mm, a.cfl_rows + (long long)slab * (gridDim.x * PH_X_ROWS) + y, tid);
     }
     lds_barrier();  // buf is reused by the forward transform
-    xr_forward<T, NX, DA, XF_FLUX3>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
+    xr_forward<T, NX, DA, FMODE>(m.fwd, NC, tw, buf, s_sig, s_k, tid, slab, y, q);
   }
 }
 

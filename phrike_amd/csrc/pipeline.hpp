@@ -212,21 +212,21 @@ void launch_xmid(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t
   PH_SWITCH_N(nx, (launch_xmid_n<T, N_, NC, FMODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int NC, int MODE>
 void launch_xend_n(int mx, dim3 grid, const XMidArgs<T>& a, hipStream_t st) {
   if (mx == N / 3 + 1) {
     check_x_pitch<T, N, true>(a.fwd.mxp);
-    hipLaunchKernelGGL((k_xend<T, N, true, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+    hipLaunchKernelGGL((k_xend<T, N, true, NC, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
   } else if (mx == N / 2 + 1) {
     check_x_pitch<T, N, false>(a.fwd.mxp);
-    hipLaunchKernelGGL((k_xend<T, N, false, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
+    hipLaunchKernelGGL((k_xend<T, N, false, NC, MODE>), grid, dim3(XLaunch<T, N>::threads), 0, st, a);
   } else {
     throw Error(-2, "unsupported x mode count");
   }
 }
-template <typename T, int MODE>
+template <typename T, int NC, int MODE>
 void launch_xend(int nx, int nrows, int nslab, const XMidArgs<T>& a, hipStream_t st) {
-  PH_SWITCH_N(nx, (launch_xend_n<T, N_, MODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
+  PH_SWITCH_N(nx, (launch_xend_n<T, N_, NC, MODE>(a.inv.mx, dim3(nrows / XLaunch<T, N_>::rows, nslab), a, st)));
 }
 
 inline void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
@@ -717,8 +717,8 @@ struct Plan3D : PlanBase {
       m.inv = a;
       m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws, gate);
       KScope ks(corr ? "xend_add" : "xend_out", st);
-      if (corr) launch_xend<T, XI_ADD>(nx, ny, nz, m, st);
-      else launch_xend<T, XI_OUT>(nx, ny, nz, m, st);
+      if (corr) launch_xend<T, 5, XI_ADD>(nx, ny, nz, m, st);
+      else launch_xend<T, 5, XI_OUT>(nx, ny, nz, m, st);
       check_launch("k_xend");
     } else {
       KScope ks(corr ? "xinv_add" : "xinv_out", st);
@@ -963,9 +963,13 @@ struct Plan2D : PlanBase {
   // boundaries fused as in the 3D step
   // cfl: also leave the per-row wave-speed maxima of the new states in
   // CflRows(ws) for max_rows()
+  // flux_in / flux_out as in Plan3D::step (chained steps of ph_run2d; the
+  // filter must be on)
   void step(int nb, const T* U, T* out, double dt, const double* dt_dev, const int* active,
-            bool filter, void* ws, hipStream_t st, bool cfl = false) const {
-    xfwd(XF_FLUX2, U, nb, 0xFu, ws, st, active);
+            bool filter, void* ws, hipStream_t st, bool cfl = false, bool flux_in = false,
+            bool flux_out = false) const {
+    if ((flux_in || flux_out) && !filter) throw Error(-9, "chained 2D steps need the post-step filter");
+    if (!flux_in) xfwd(XF_FLUX2, U, nb, 0xFu, ws, st, active);
     for (int s = 1; s <= 4; ++s) {
       ypass(true, 0, nb, ws, st, active);
       XMidArgs<T> m{};
@@ -995,8 +999,16 @@ struct Plan2D : PlanBase {
     a.out = out;
     a.active = active;
     a.cfl_rows = cfl ? CflRows(ws) : nullptr;
-    launch_xinv<T, 4, XI_OUT>(nx, ny, nb, a, st);
-    check_launch("k_xinv(2d filter)");
+    if (flux_out) {
+      XMidArgs<T> m{};
+      m.inv = a;
+      m.fwd = xfwd_args(0xFu, ws);
+      launch_xend<T, 4, XI_OUT>(nx, ny, nb, m, st);
+      check_launch("k_xend(2d)");
+    } else {
+      launch_xinv<T, 4, XI_OUT>(nx, ny, nb, a, st);
+      check_launch("k_xinv(2d filter)");
+    }
   }
   // max and dt per run of the states written by the last step(..., cfl = true)
   void max_rows(int nb, double* out, double* dt_out, double cfl, void* ws, hipStream_t st,

@@ -371,3 +371,37 @@ def test_chained_steps(emu):
         np.testing.assert_array_equal(C, D)
     with pytest.raises(RuntimeError, match="flags"):
         e.step(U0.astype(np.float32), dt, 16)
+
+
+def test_run2d_chained_steps(emu):
+    """ph_run2d chains its steps (each but the last forms the next step's
+    fluxes in k_xend): one call of 3 steps equals three calls of one step bit
+    for bit (states, scalars, flags), with one run inactive throughout and one
+    that stops after its second (clipped) step."""
+    from emu_lib import Emu2D, P
+    from phrike_amd.initial_conditions import khi_2d_periodic
+    cfl, nb = 0.4, 3
+    e = Emu2D(emu, 32, 16, batch=nb, dtype=np.float32)
+    U0 = np.ascontiguousarray(np.stack([khi_2d_periodic(32, 16, seed=s + 3) for s in range(nb)]).astype(np.float32))
+    _, d0 = e.dt(U0, cfl)
+
+    def fresh():
+        rs = np.zeros(6 * nb)
+        rf = np.zeros(2 * nb, dtype=np.int32)
+        rf[:nb] = [1, 0, 1]
+        rs[nb:2 * nb] = d0
+        rs[2 * nb:3 * nb] = [1.0, 1.0, 1.5 * d0[2]]
+        rs[3 * nb:4 * nb] = np.inf
+        return U0.copy(), np.zeros_like(U0), rs, rf
+
+    U, V, rs, rf = fresh()
+    assert emu.ph_run2d(e.h, nb, P(U), P(V), 3, P(rs), P(rf), cfl, P(e.ws), None) == 0
+    U1, V1, rs1, rf1 = fresh()
+    for k in range(3):
+        a, b = (U1, V1) if k % 2 == 0 else (V1, U1)
+        assert emu.ph_run2d(e.h, nb, P(a), P(b), 1, P(rs1), P(rf1), cfl, P(e.ws), None) == 0
+    np.testing.assert_array_equal(rf, rf1)
+    assert list(rf[nb:]) == [3, 0, 2]
+    np.testing.assert_array_equal(rs, rs1)
+    np.testing.assert_array_equal(U, U1)
+    np.testing.assert_array_equal(V, V1)
