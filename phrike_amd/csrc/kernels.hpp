@@ -53,6 +53,11 @@ struct ColElem<float> {
 };
 #endif
 
+// 1/v, or 1 where v == 0 (per lane for a column pair)
+__device__ __forceinline__ double rcp_or_one(double v) { return v == 0.0 ? 1.0 : 1.0 / v; }
+__device__ __forceinline__ float rcp_or_one(float v) { return v == 0.f ? 1.f : 1.f / v; }
+__device__ __forceinline__ f2 rcp_or_one(f2 v) { return f2(rcp_or_one(v.a), rcp_or_one(v.b)); }
+
 // ----------------------------------------------------------------------------
 // retained-mode bookkeeping along one axis: positions n <= kp and
 // n >= N - kn are kept; compact index j = n (n <= kp) or n - (N - nr).
@@ -225,7 +230,9 @@ enum ColMode {
   CM_FWDINV_DIV = 2,  // forward both halves, D = sigma (X + i k Z), inverse, full out
   CM_FWDINV_COPY = 3, // forward, sigma (and i*k) on kept modes, inverse, full out
   CM_INV = 4,         // compact in, zero-filled inverse, full out
-  CM_FWDINV_CORR = 5  // forward, (sx sy sz - 1) on kept modes, inverse, full out
+  CM_FWDINV_CORR = 5, // forward, (sx sy sz - 1) on kept modes, inverse, full out
+  CM_FWDINV_PROJ = 6  // three fields (B) together: forward, sigma and the Leray
+                      // projection on kept modes, inverse, full out
 };
 
 template <typename T>
@@ -250,6 +257,10 @@ struct ColArgs {
   const T* sm1;
   const T* colsm1;
   const T* rowsm1;
+  // CM_FWDINV_PROJ: k of the column axis (by column) and of the axis of
+  // blockIdx.y (by blockIdx.y); this axis's k is kk
+  const T* colk;
+  const T* rowk;
 };
 
 #ifndef PH_COL_ELEMS_F64
@@ -262,6 +273,12 @@ struct ColArgs {
 // take twice the columns, so a row segment is 128 bytes
 #ifndef PH_COL_ELEMS_F32_SINGLE
 #define PH_COL_ELEMS_F32_SINGLE 8192
+#endif
+
+// the projection mode holds three spectra of a tile in LDS (two besides the
+// transform buffer): 2048 elements per spectrum keep it at ~100 KB per block
+#ifndef PH_COL_ELEMS_PROJ
+#define PH_COL_ELEMS_PROJ 2048
 #endif
 
 // Two-input modes hold the first spectrum in registers (YREG) or in a second
@@ -284,8 +301,10 @@ template <typename T, int N, int MODE>
 struct ColTile {
   static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
   static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
-  static constexpr int ELEMS =
-      (sizeof(T) == 8) ? PH_COL_ELEMS_F64 : (PAIR ? PH_COL_ELEMS_F32 : PH_COL_ELEMS_F32_SINGLE);
+  static constexpr int ELEMS = MODE == CM_FWDINV_PROJ
+                                   ? PH_COL_ELEMS_PROJ
+                                   : ((sizeof(T) == 8) ? PH_COL_ELEMS_F64
+                                                       : (PAIR ? PH_COL_ELEMS_F32 : PH_COL_ELEMS_F32_SINGLE));
   // columns per tile: PH_COL_TM_MAX 16-byte elements at most (128-byte rows),
   // so short lines make more, smaller blocks rather than wider rows
   static constexpr int TM = clampi(ELEMS / N < PH_COL_TM_MAX ? ELEMS / N : PH_COL_TM_MAX, 1, 64);
@@ -330,6 +349,8 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   constexpr bool PAIR = ColTile<T, N, MODE>::PAIR;
   constexpr bool YREG = ColTile<T, N, MODE>::YREG;
   __shared__ cx<T> ybuf[(PAIR && !YREG) ? N * TM : 1];
+  // CM_FWDINV_PROJ: the x and y components' spectra while the z one is transformed
+  __shared__ cx<T> pbuf[MODE == CM_FWDINV_PROJ ? 2 * N * TM : 1];
   cx<T> Y[YREG ? PER : 1][R];
   auto first = [&](int q, int r, int l, int n) {
     if constexpr (YREG) return Y[q][r];
@@ -422,6 +443,53 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           st_stream(o2 + (long long)j * ols + l, cscale(X[q][r], a.sig[j]));
         }
       });
+    } else if constexpr (MODE == CM_FWDINV_PROJ) {
+      // nc = 3: this iteration does all three fields.  Forward transforms
+      // of the x and y components wait in pbuf (each thread reads back only
+      // the slots it wrote), the z component stays in X; on kept modes
+      // sigma, then B^ -= k (k.B^)/|k|^2 (|k|^2 = 1 at k = 0); the inverse
+      // transforms run in place in buf, pbuf and pbuf + N TM.
+      cx<T>* pb0 = pbuf;
+      cx<T>* pb1 = pbuf + N * TM;
+      auto keep_to = [&](cx<T>* dst) {
+        each([&](int q, int r, int l, int n, bool ok) { dst[n * TM + l] = X[q][r]; });
+      };
+      fwd(FWD{});
+      keep_to(pb0);
+      fwd(FWD{});
+      keep_to(pb1);
+      fwd(FWD{});
+      const T ky = a.rowk[blockIdx.y];
+      each([&](int q, int r, int l, int n, bool ok) {
+        cx<T> bx = mk(T(0), T(0)), by = bx, bz = bx;
+        if (kp.kept(n, N)) {
+          const int j = kp.compact(n, N);
+          const T sg = a.sig[j], kx = a.colk[c0 + l], kz = a.kk[j];
+          bx = cscale(pb0[n * TM + l], sg);
+          by = cscale(pb1[n * TM + l], sg);
+          bz = cscale(X[q][r], sg);
+          const T k2 = (kx * kx + ky * ky) + kz * kz;
+          const cx<T> kb = mk((kx * bx.x + ky * by.x) + kz * bz.x, (kx * bx.y + ky * by.y) + kz * bz.y);
+          const T inv = rcp_or_one(k2);
+          const cx<T> cf = mk(kb.x * inv, kb.y * inv);
+          bx = bx - cscale(cf, kx);
+          by = by - cscale(cf, ky);
+          bz = bz - cscale(cf, kz);
+        }
+        pb0[n * TM + l] = bx;
+        pb1[n * TM + l] = by;
+        buf[n * TM + l] = bz;
+      });
+      lds_barrier();
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        fft_regs<T, N, TM, +1, NT>(f == 0 ? pb0 : (f == 1 ? pb1 : buf), tw, tid, X);
+        cx<T>* of = out + (long long)f * a.out_fs + c0;
+        each([&](int q, int r, int l, int n, bool ok) {
+          if (ok) st_stream(of + (long long)n * ols + l, X[q][r]);
+        });
+      }
+      break;
     } else {
       // CM_FWDINV_DIV / CM_FWDINV_COPY / CM_FWDINV_CORR: forward, combine on
       // kept modes, inverse

@@ -317,9 +317,19 @@ struct PlanMHD3D : PlanBase {
       check_launch("k_xmid(mhd)");
     }
     // the filter's y, z and y-inverse passes of the 8 fields; the x inverse
-    // writes the filtered state
+    // writes the filtered state.  With the 2/3 rule (no Nyquist mode kept)
+    // the final projection runs inside the filter's z pass: B^ of the kept
+    // modes is projected while its three components are in LDS, and E keeps
+    // its filtered value -- the reference recomputes E at fixed thermal
+    // pressure, E + (|P B|^2 - |B|^2)/2, and B here is the filtered state of
+    // a solenoidal step, so P B - B and that difference are round-off.
     main.yfwd(false, 8, 0, w, st);
-    main.zpass(false, 8, 0, w, st);
+    if (proj_in_filter()) {
+      main.zpass(false, 5, 0, w, st);
+      main.zproj(5, w, st);
+    } else {
+      main.zpass(false, 8, 0, w, st);
+    }
     main.yinv(8, w, st);
     {
       XInvArgs<T> a = main.xinv_args(w);
@@ -329,8 +339,10 @@ struct PlanMHD3D : PlanBase {
       launch_xinv<T, 8, XI_OUT>(nx, ny, nz, a, st);
       check_launch("k_xinv(mhd filter)");
     }
-    project(out, ws, st);
+    if (!proj_in_filter()) project(out, ws, st);
   }
+  // the main plan keeps no Nyquist mode (2/3 rule along every axis)
+  bool proj_in_filter() const { return main.ax.kp < nx / 2 && main.ay.kp < ny / 2 && main.az.kp < nz / 2; }
   // phrike/solver.py:988-1006, 1041-1050
   void step(const T* U, T* out, double dt, void* ws, hipStream_t st) const {
     T* us = Us(ws);

@@ -349,6 +349,7 @@ struct Plan3D : PlanBase {
   int mx, mxp;
   long long ncell;
   CE* xcol_sm1 = nullptr;  // sigma_x - 1 per column of the column passes
+  CE* xcol_k = nullptr;    // kx per column of the column passes (zero past the kept modes)
   T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
   bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
   // workspace layout (bytes)
@@ -389,6 +390,16 @@ struct Plan3D : PlanBase {
       }
       hip_check(hipMalloc(&xcol_sm1, sizeof(CE) * ncolw), "hipMalloc(column sigma-1)");
       hip_check(hipMemcpy(xcol_sm1, h.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column sigma-1");
+      std::vector<CE> hk(ncolw, CE(0.0));
+      for (int c = 0; c < ncol; ++c) {
+        if constexpr (PW == 2) {
+          hk[c] = CE(float(ax.k[2 * c]), float(2 * c + 1 < mx ? ax.k[2 * c + 1] : 0.0));
+        } else {
+          hk[c] = CE(ax.k[c]);
+        }
+      }
+      hip_check(hipMalloc(&xcol_k, sizeof(CE) * ncolw), "hipMalloc(column k)");
+      hip_check(hipMemcpy(xcol_k, hk.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column k");
     }
     {
       std::vector<T> ones(mx, T(1));
@@ -409,6 +420,7 @@ struct Plan3D : PlanBase {
   }
   ~Plan3D() override {
     if (xcol_sm1) (void)hipFree(xcol_sm1);
+    if (xcol_k) (void)hipFree(xcol_k);
     if (unit_x) (void)hipFree(unit_x);
     dx.release();
     dy.release();
@@ -553,6 +565,32 @@ struct Plan3D : PlanBase {
     if (fwd) launch_col<CE, CM_FWD_COPY>(nz, ay.nr, a, st);
     else launch_col<CE, CM_INV>(nz, ay.nr, a, st);
     check_launch("k_col(z line)");
+  }
+  // fields f0..f0+2 of B (a solenoidal field's three components) along z:
+  // forward, sigma_z and the Leray projection on kept modes, inverse, in place
+  void zproj(int f0, void* ws, hipStream_t st) const {
+    ColArgs<CE> a{};
+    cx<CE>* b = reinterpret_cast<cx<CE>*>(B(ws)) + (long long)f0 * (b_fs() / PW);
+    a.in = b;
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
+    a.out = b;
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
+    a.nc = 3;
+    a.keep = az.keep();
+    a.tw = dz.tw;
+    a.sig = dz.sig;
+    a.kk = dz.k;
+    a.colk = xcol_k;
+    a.rowk = dy.k;
+    KScope ks("z_proj", st);
+    launch_col<CE, CM_FWDINV_PROJ>(nz, ay.nr, a, st);
+    check_launch("k_col(z projection)");
   }
   // filter correction along z: kept modes times sx sy sz - 1 (after unit x
   // and y passes), in place in B
