@@ -383,8 +383,9 @@ int ph_step_mhd3d(ph_planmhd3d_t plan, const void* U, void* out, double dt, void
  * spectral factor is common to the three components), so every stage state
  * keeps a solenoidal B and the projection after each substage is the
  * identity up to round-off: the step runs its stage boundaries fused, as the
- * Euler step does, and projects once after the filter.  Results equal the
- * unflagged step to round-off.  Without the flag: ph_step_mhd3d.  Replaces
+ * Euler step does, and projects once, inside the filter's z pass (with the
+ * 2/3 rule; else after the filter).  Results equal the unflagged step to
+ * round-off.  Without the flag: ph_step_mhd3d.  Replaces
  * the same routines as ph_step_mhd3d (phrike/solver.py:956-1006, 1037-1050). */
 #define PH_STEP_SOLENOIDAL 2
 int ph_step_mhd3d_ex(ph_planmhd3d_t plan, const void* U, void* out, double dt, int flags, void* ws,

@@ -18,7 +18,8 @@
 //          unprojected state.
 //   filter the main plan's spectral filter of all 8 fields.
 //   fused  for a solenoidal U (PH_STEP_SOLENOIDAL): the stage boundaries as
-//          one k_xmid each, one projection per step (step_fused).
+//          one k_xmid each, one projection per step, inside the filter's z
+//          pass (step_fused).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
