@@ -219,6 +219,26 @@ template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG
 template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
 template <> struct XFwdShape<XF_COPY8> { static constexpr int NC = 8, LX = 8, NG = 1; };
 
+// The Euler flux transforms of a row in one FFT over all NG*LX lines instead
+// of NG groups of LX (FP32 rows up to PH_XF_MERGE_F32_MAXNX points, FP64 up to
+// PH_XF_MERGE_F64_MAXNX): the radix passes keep ~94 % of the lanes busy
+// instead of 62-83 % and take a third of the barriers, for NG times the LDS.
+// Off by default: at 256^3 FP32 the kernels then need 139 VGPRs and the step
+// took 4.47 instead of 4.19 ms.
+#ifndef PH_XF_MERGE_F32_MAXNX
+#define PH_XF_MERGE_F32_MAXNX 0
+#endif
+#ifndef PH_XF_MERGE_F64_MAXNX
+#define PH_XF_MERGE_F64_MAXNX 0
+#endif
+template <typename T, int NX, int MODE>
+struct XMerge {
+  static constexpr bool value = (MODE == XF_FLUX3 || MODE == XF_FLUX2) &&
+                                NX <= (sizeof(T) == 4 ? PH_XF_MERGE_F32_MAXNX : PH_XF_MERGE_F64_MAXNX);
+  // LDS lines of the forward transform
+  static constexpr int lines = value ? XFwdShape<MODE>::LX * XFwdShape<MODE>::NG : XFwdShape<MODE>::LX;
+};
+
 
 // ----------------------------------------------------------------------------
 // column passes (y and z): tiles of TM consecutive mx columns, lines along the
@@ -891,11 +911,12 @@ __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
 }
 
 // Z (M-point forward transforms of LX packed lines, in LDS) -> kept modes
-// X[k] = (E + W^k O) * sig[k] (* i k[k]) of lines l < nl, stored as 16-byte
-// chunks to dst (one row of the line's field; field stride fs)
-template <typename T, int NX, bool DA, int LX>
+// X[k] = (E + W^k O) * sig[k] (* i k[k] where bit (l / GL) * GL of ikmask is
+// set) of lines l < nl, stored as 16-byte chunks to dst (one row of the
+// line's field; field stride fs)
+template <typename T, int NX, bool DA, int LX, int GL = LX>
 __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw, cx<T>* dst, long long fs, int nl,
-                                               const T* sig, const T* kk, bool ik, int tid) {
+                                               const T* sig, const T* kk, unsigned ikmask, int tid) {
   using G = XRGeo<T, NX, DA>;
   using C = typename ChunkT<T, G::K>::type;
   constexpr int NTASK = LX * G::MCP, M = G::M;
@@ -915,13 +936,28 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
           const cx<T> h = mk(T(0.5) * (a.x - z2.x), T(0.5) * (a.y + z2.y));  // (Z[k] - Z*[M-k]) / 2
           cx<T> X = e + cmul(tw[k], mk(h.y, -h.x));                           // E + W^k (-i h)
           X = cscale(X, sig[k]);
-          if (ik) X = mul_ik(X, kk[k]);
+          if ((ikmask >> ((l / GL) * GL)) & 1u) X = mul_ik(X, kk[k]);  // i k of line l's group
           ch_set(c, i, X);
         }
       }
       st_stream(d + l * fsc + m, c);
     }
   }
+}
+
+// Euler flux of direction g, component l from the primitives w = (rho, ux,
+// uy, uz, p, E + p) (phrike/equations.py:119-134, 167-196); the direction is
+// picked by selects (a register index by a run-time g would put w in scratch
+// memory)
+template <typename T, int NC>
+__device__ __forceinline__ T euler_flux(const T* w, int g, int l) {
+  const T ud = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
+  const T md = w[0] * ud;
+  if (l == 0) return md;
+  if (l == NC - 1) return w[5] * ud;
+  T v = md * w[l];
+  if (l == g + 1) v = v + w[4];
+  return v;
 }
 
 // Fluxes (or copies) of the conserved values q at the thread's x pairs ->
@@ -984,6 +1020,26 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
       }
   }
   cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y * a.mxp;
+  if constexpr (XMerge<T, NX, MODE>::value) {
+    // all NG directions' fluxes as one batch of lines (group g = lines g*LX..)
+    constexpr int LG = XMerge<T, NX, MODE>::lines;
+#pragma unroll
+    for (int g = 0; g < S::NG; ++g)
+#pragma unroll
+      for (int j = 0; j < PJ; ++j) {
+        const int p = G::point(tid, j);
+        if (G::PALL || p < M) {
+#pragma unroll
+          for (int l = 0; l < LX; ++l)
+            buf[xslot<LG>(p, g * LX + l)] = mk(euler_flux<T, NC>(pr[j][0], g, l), euler_flux<T, NC>(pr[j][1], g, l));
+        }
+      }
+    lds_barrier();
+    fft_lds<T, M, LG, -1, NT, xstride(LG), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
+    xr_split_store<T, NX, DA, LG, LX>(buf, tw, dst, a.out_fs, LG, sig, kk, a.ikmask, tid);
+    lds_barrier();
+    return;
+  }
   const int nl = xf_copy(MODE) ? ncomp : LX;
 #pragma unroll 1
   for (int g = 0; g < S::NG; ++g) {
@@ -1014,16 +1070,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
               } else if (l == 4) v[r] = w[10] * ug - Bg * w[9];
               else v[r] = (l - 5 == g) ? T(0) : ug * w[l] - w[l - 4] * Bg;
             } else {
-              // flux of direction g, component l (phrike/equations.py:119-134, 167-196)
-              const T* w = pr[j][r];
-              const T ud = g == 0 ? w[1] : (g == 1 ? w[2] : w[3]);
-              const T md = w[0] * ud;
-              if (l == 0) v[r] = md;
-              else if (l == NC - 1) v[r] = w[5] * ud;
-              else {
-                v[r] = md * w[l];
-                if (l == g + 1) v[r] = v[r] + w[4];
-              }
+              v[r] = euler_flux<T, NC>(pr[j][r], g, l);
             }
           }
           buf[xslot<LX>(p, l)] = mk(v[0], v[1]);
@@ -1044,7 +1091,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   using S = XFwdShape<MODE>;
   constexpr int NT = G::NT, NC = S::NC, PJ = G::PJ;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xrbuf_size<NX, S::LX>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, XMerge<T, NX, MODE>::lines>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
@@ -1321,7 +1368,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   constexpr int LMODE = NC == 8 ? XF_COPY8 : XF_COPY;
   constexpr int QN = XFwdShape<LMODE>::NC;
   PH_XTW_DECL(T, NX);
-  constexpr int LB0 = NC > XFwdShape<FMODE>::LX ? NC : XFwdShape<FMODE>::LX;
+  constexpr int LB0 = NC > XMerge<T, NX, FMODE>::lines ? NC : XMerge<T, NX, FMODE>::lines;
   constexpr int LB = LB0 > XFwdShape<LMODE>::LX ? LB0 : XFwdShape<LMODE>::LX;
   __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
@@ -1444,8 +1491,9 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   constexpr int FMODE = NC == 5 ? XF_FLUX3 : XF_FLUX2;  // 3D or batched-2D Euler state
   static_assert(NC == 4 || NC == 5, "Euler states");
   static_assert(MODE == XI_OUT || MODE == XI_ADD, "the filter's last pass writes or adds");
+  constexpr int LB = NC > XMerge<T, NX, FMODE>::lines ? NC : XMerge<T, NX, FMODE>::lines;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xrbuf_size<NX, NC>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
