@@ -363,6 +363,18 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   const int ncol = a.ncol;
   const int ncolw = a.ncolw > 0 ? a.ncolw : ncol;
   const int nc = a.nc;
+  // tile rows of a whole 128-byte line stream (non-temporal); narrower rows
+  // (long lines: TM < 8) keep cached accesses, so the two halves of a line,
+  // read and written by neighbouring tiles, meet in L2 rather than in HBM
+  constexpr bool LINE = TM * sizeof(cx<T>) >= 128;
+  auto ld = [](const cx<T>* p) {
+    if constexpr (LINE) return ld_stream(p);
+    else return *p;
+  };
+  auto st = [](cx<T>* p, const cx<T>& v) {
+    if constexpr (LINE) st_stream(p, v);
+    else *p = v;
+  };
   cx<T> X[PER][R];
   cx<T> W[P0::PER][P0::R];
   // first spectrum of a pair, at the kept positions this thread owns
@@ -390,10 +402,10 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
     return [=](int l, int n) {
       if constexpr (MODE == CM_INV)
-        return (c0 + l < ncol && kp.kept(n, N)) ? ld_stream(src + (long long)kp.compact(n, N) * ls + l)
+        return (c0 + l < ncol && kp.kept(n, N)) ? ld(src + (long long)kp.compact(n, N) * ls + l)
                                                 : mk(T(0), T(0));
       else
-        return (c0 + l < ncol) ? ld_stream(src + (long long)n * ls + l) : mk(T(0), T(0));
+        return (c0 + l < ncol) ? ld(src + (long long)n * ls + l) : mk(T(0), T(0));
     };
   };
   // visit every (q, r) output slot of this thread: f(q, r, l, n, col_ok)
@@ -432,7 +444,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     if constexpr (MODE == CM_INV) {
       fwd(INV{});
       each([&](int q, int r, int l, int n, bool ok) {
-        if (ok) st_stream(oc + (long long)n * ols + l, X[q][r]);
+        if (ok) st(oc + (long long)n * ols + l, X[q][r]);
       });
     } else if constexpr (MODE == CM_FWD_COPY) {
       fwd(FWD{});
@@ -441,7 +453,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           const int j = kp.compact(n, N);
           cx<T> v = cscale(X[q][r], a.sig[j]);
           if (a.ikflag) v = mul_ik(v, a.kk[j]);
-          st_stream(oc + (long long)j * ols + l, v);
+          st(oc + (long long)j * ols + l, v);
         }
       });
     } else if constexpr (MODE == CM_FWD_DIV3) {
@@ -452,7 +464,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          st_stream(oc + (long long)j * ols + l, cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
+          st(oc + (long long)j * ols + l, cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
         }
       });
       cx<T>* o2 = out + (long long)(nc + c) * a.out_fs + c0;
@@ -460,7 +472,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          st_stream(o2 + (long long)j * ols + l, cscale(X[q][r], a.sig[j]));
+          st(o2 + (long long)j * ols + l, cscale(X[q][r], a.sig[j]));
         }
       });
     } else if constexpr (MODE == CM_FWDINV_PROJ) {
@@ -506,7 +518,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         fft_regs<T, N, TM, +1, NT>(f == 0 ? pb0 : (f == 1 ? pb1 : buf), tw, tid, X);
         cx<T>* of = out + (long long)f * a.out_fs + c0;
         each([&](int q, int r, int l, int n, bool ok) {
-          if (ok) st_stream(of + (long long)n * ols + l, X[q][r]);
+          if (ok) st(of + (long long)n * ols + l, X[q][r]);
         });
       }
       break;
@@ -539,7 +551,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       lds_barrier();
       fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
       each([&](int q, int r, int l, int n, bool ok) {
-        if (ok) st_stream(oc + (long long)n * ols + l, X[q][r]);
+        if (ok) st(oc + (long long)n * ols + l, X[q][r]);
       });
     }
   }
