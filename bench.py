@@ -529,6 +529,14 @@ def _ensemble_summary(res, wall, sizes):
                            "p99_s": round(float(np.percentile(lat, 99)), 3)}
     out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}
     out["per_size"] = per
+    # the 2D pencil-pipeline byte model C2(N) = 80 + 112 a per run-step (FP32)
+    # over the wall time, against 8 TB/s
+    moved = 0.0
+    for r in res:
+        a = 2 * (r.n // 3 + 1) / r.n
+        moved += r.steps * (80 + 112 * a) * 4 * r.n ** 2
+    out["model_GBs"] = round(moved / wall / 1e9, 1)
+    out["hbm_frac_model"] = round(moved / wall / HBM_PEAK, 4)
     return out
 
 
@@ -934,6 +942,8 @@ def main():
         "step_fused_bytes": int(fused_step_units(n, args.precision) * es * n ** 3),
         "hbm_frac_step_fused": round(fused_step_units(n, args.precision) * es * n ** 3 / (ms_per_step / 1e3)
                                      / HBM_PEAK, 4),
+        # the dense (unpruned) pencil model, C = 420, for transparency
+        "hbm_frac_step_dense": round(420 * es * n ** 3 / (ms_per_step / 1e3) / HBM_PEAK, 4),
         "wall_s": round(wall, 3),
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels,
         "mass_drift": float(sums[0] / n ** 3 - 1.0),
