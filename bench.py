@@ -324,6 +324,15 @@ def hybrid_study(nx: int, nys, steps: int, dev, ensembles=(16, 64), check=200):
             ms[name] = (time.perf_counter() - t0) / steps * 1e3
             if name != "f64_perturbed":
                 dk[name] = kernel_rate(sol, U0, dt, ny)
+                # the same steps replayed from a captured HIP graph of 50 steps
+                sol.advance(U0, 50, dt=dt, graph_steps=50)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                Ug = sol.advance(U0, steps, dt=dt, graph_steps=50)
+                torch.cuda.synchronize()
+                ms[name + "_graph"] = (time.perf_counter() - t0) / steps * 1e3
+                if not torch.equal(Ug.to(torch.float64), U2.to(torch.float64)):
+                    raise RuntimeError(f"hybrid {name}: graph-replayed steps differ from eager steps")
             snaps[name] = (U1.double(), U2.double())
             del g, sol
         ref = snaps["f64"]

@@ -183,3 +183,27 @@ def test_hybrid_c4_ny256_vs_oracle(dev):
     e = field_errors(Ut.cpu().numpy(), Uo)
     print(f"hybrid 256x256 3 steps errors {e}", flush=True)
     assert e.max() < 1e-11
+
+
+def test_hybrid_graph_replay_equals_eager(dev):
+    """advance(..., graph_steps=4): the steps replayed from a captured HIP
+    graph give the eager steps' states bit for bit, over several replays with
+    eager work and synchronisations between them, FP64 and split-FP32 forms,
+    fixed dt and the per-step device dt."""
+    import torch
+    from phrike_amd.initial_conditions import khi_2d_gll
+    for prec, gll in (("double", "f64"), ("single", "split")):
+        og, osol, g, sol = _setup(64, 32, dev, precision=prec, gll=gll, batch=2)
+        U0 = torch.stack([torch.as_tensor(khi_2d_gll(64, g.y_nodes, seed=3 + i), device=dev)
+                          for i in range(2)]).to(g.dtype)
+        dt = sol.compute_dt(U0[0])
+        for fixed in (dt, None):
+            print("case graph replay", prec, gll, "fixed dt" if fixed else "device dt", flush=True)
+            ref = sol.advance(U0, 13, dt=fixed)
+            got = sol.advance(U0, 13, dt=fixed, graph_steps=4)  # 3 replays + 1 eager step
+            torch.cuda.synchronize()
+            assert torch.equal(ref, got)
+            extra = sol.step(got, dt) * 0.5  # eager work between captures and replays
+            torch.cuda.synchronize()
+            again = sol.advance(U0, 13, dt=fixed, graph_steps=4)
+            assert torch.equal(ref, again) and torch.isfinite(extra).all()
