@@ -35,13 +35,15 @@ def byte_model(n: int):
     k = n // 3
     a = 2 * (k + 1) / n
     b = 2 * (k + 1) * (2 * k + 1) / n ** 2
+    # the x kernels write the 12 distinct Euler fluxes (the momentum flux
+    # tensor is symmetric), which the y pass reads
     per = {
-        "xfwd_flux": 5 + 15 * a, "yfwd_div": 15 * a + 10 * b, "z_div": 15 * b,
+        "xfwd_flux": 5 + 12 * a, "yfwd_div": 12 * a + 10 * b, "z_div": 15 * b,
         "yinv": 5 * b + 5 * a, "xinv_stage1": 5 * a + 15, "xinv_stage23": 5 * a + 20,
         "xinv_stage4": 5 * a + 10, "xfwd_copy": 5 + 5 * a, "yfwd_copy": 5 * a + 5 * b,
         "z_copy": 10 * b, "xinv_out": 5 * a + 5,
         # fused stage boundaries: D in, U in, accumulator, next-stage spectra out
-        "xmid1": 5 * a + 5 + 5 + 15 * a, "xmid23": 5 * a + 5 + 10 + 15 * a,
+        "xmid1": 5 * a + 5 + 5 + 12 * a, "xmid23": 5 * a + 5 + 10 + 12 * a,
         # (stage 4 reads the accumulator, which carries U, and not U itself)
         "xmid_last": 5 * a + 5 + 5 * a,
         # FP32 filter in correction form: stage 4 also writes the new state,
@@ -50,7 +52,7 @@ def byte_model(n: int):
         "z_corr": 10 * b, "xinv_add": 5 * a + 5 + 5,
         # chained steps: the filter's x inverse also forms the next step's
         # fluxes and their x transform (the state is not read again)
-        "xend_out": 5 * a + 5 + 15 * a, "xend_add": 5 * a + 5 + 5 + 15 * a,
+        "xend_out": 5 * a + 5 + 12 * a, "xend_add": 5 * a + 5 + 5 + 12 * a,
     }
     # per-step model of the unfused pencil pipeline (stage states written and
     # re-read), the official figure for hbm_frac_step

@@ -58,6 +58,17 @@ __device__ __forceinline__ double rcp_or_one(double v) { return v == 0.0 ? 1.0 :
 __device__ __forceinline__ float rcp_or_one(float v) { return v == 0.f ? 1.f : 1.f / v; }
 __device__ __forceinline__ f2 rcp_or_one(f2 v) { return f2(rcp_or_one(v.a), rcp_or_one(v.b)); }
 
+// Spectral field holding the 3D Euler flux F_d[c] in the x pass output of
+// the symmetric layout (the 12 distinct fluxes, euler3_flux_sym; d = 0, 1, 2
+// for x, y, z; c = 0..4), 4 bits per (c, d) at bit 4 (3 c + d)
+constexpr unsigned long long kEulerSymField =
+    (0ull << 0) | (1ull << 4) | (2ull << 8) |        // rho:  u0 u1 u2
+    (3ull << 12) | (6ull << 16) | (7ull << 20) |     // rho u: u3 u6 u7
+    (6ull << 24) | (4ull << 28) | (8ull << 32) |     // rho v: u6 u4 u8
+    (7ull << 36) | (8ull << 40) | (5ull << 44) |     // rho w: u7 u8 u5
+    (9ull << 48) | (10ull << 52) | (11ull << 56);    // E:     u9 u10 u11
+__device__ __forceinline__ int sym_field(int c, int d) { return int((kEulerSymField >> (4 * (3 * c + d))) & 15ull); }
+
 // ----------------------------------------------------------------------------
 // retained-mode bookkeeping along one axis: positions n <= kp and
 // n >= N - kn are kept; compact index j = n (n <= kp) or n - (N - nr).
@@ -213,7 +224,9 @@ __device__ __forceinline__ bool gated_off(const int* active, int act_all, int sl
 }
 
 template <int MODE> struct XFwdShape;
-template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 5, NG = 3; };
+// 3D Euler: the 12 distinct fluxes (the momentum flux tensor is symmetric),
+// 3 groups of 4 lines; see euler3_flux_sym
+template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 4, NG = 3; };
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
@@ -281,6 +294,9 @@ struct ColArgs {
   // blockIdx.y (by blockIdx.y); this axis's k is kk
   const T* colk;
   const T* rowk;
+  // CM_FWD_DIV3 with the 12-field symmetric 3D Euler flux layout
+  // (sym_field): F_x[c] is read without i kx, which is applied here (colk)
+  int symflux;
 };
 
 #ifndef PH_COL_ELEMS_F64
@@ -389,9 +405,14 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     else return ybuf[n * TM + l];
   };
 
+  // CM_FWD_DIV3, symmetric layout: kx of the tile's columns
+  __shared__ T s_colk[MODE == CM_FWD_DIV3 ? TM : 1];
+  if constexpr (MODE == CM_FWD_DIV3) {
+    if (a.symflux && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
+  }
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
-    if constexpr (MODE == CM_FWD_DIV3) return (k % 3) * nc + k / 3;
+    if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_field(k / 3, k % 3) : (k % 3) * nc + k / 3;
     else if constexpr (MODE == CM_FWDINV_DIV) return (k % 2) * nc + k / 2;
     else return k;
   };
@@ -399,13 +420,18 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   const int nsrc = PERC * nc;
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
-    const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
+    const int f = field_of(k);
+    const cx<T>* src = in + (long long)f * a.in_fs + c0;
+    // the symmetric layout's shared fluxes (fields 6-8) are read twice per
+    // tile: cached loads, so the second read can come from the caches
+    const bool shared = MODE == CM_FWD_DIV3 && a.symflux && f >= 6 && f <= 8;
     return [=](int l, int n) {
       if constexpr (MODE == CM_INV)
         return (c0 + l < ncol && kp.kept(n, N)) ? ld(src + (long long)kp.compact(n, N) * ls + l)
                                                 : mk(T(0), T(0));
       else
-        return (c0 + l < ncol) ? ld(src + (long long)n * ls + l) : mk(T(0), T(0));
+        return (c0 + l < ncol) ? (shared ? src[(long long)n * ls + l] : ld(src + (long long)n * ls + l))
+                               : mk(T(0), T(0));
     };
   };
   // visit every (q, r) output slot of this thread: f(q, r, l, n, col_ok)
@@ -457,14 +483,17 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         }
       });
     } else if constexpr (MODE == CM_FWD_DIV3) {
-      // G = sigma (X + i k Z) with X = F(Fx^), Z = F(Fy^); then sigma F(Fz^)
+      // G = sigma (X + i k Z) with X = F(i kx Fx^), Z = F(Fy^); then sigma
+      // F(Fz^).  The symmetric layout stores Fx^ without i kx: applied here.
       fwd(FWD{});
       keep_first();
       fwd(FWD{});
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          st(oc + (long long)j * ols + l, cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
+          const cx<T> x0 = first(q, r, l, n);
+          const cx<T> x = a.symflux ? mul_ik(x0, s_colk[l]) : x0;
+          st(oc + (long long)j * ols + l, cscale(x + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
         }
       });
       cx<T>* o2 = out + (long long)(nc + c) * a.out_fs + c0;
@@ -972,6 +1001,27 @@ __device__ __forceinline__ T euler_flux(const T* w, int g, int l) {
   return v;
 }
 
+// The 12 distinct 3D Euler fluxes from the primitives w = (rho, ux, uy, uz, p,
+// E + p) (phrike/equations.py:167-196).  Of the 15 fluxes F_d[c], the mass
+// fluxes are rho u_d and the momentum fluxes rho u_i u_j + p delta_ij are
+// symmetric in (i, j), so F_x[rho v] = F_y[rho u] etc.:
+//   u = 0..2   rho ux, rho uy, rho uz              (F_x,y,z[rho])
+//   u = 3..5   (rho ux) ux + p, (rho uy) uy + p, (rho uz) uz + p
+//   u = 6..8   (rho ux) uy, (rho ux) uz, (rho uy) uz
+//   u = 9..11  (E + p) ux, (E + p) uy, (E + p) uz   (F_x,y,z[E])
+// Each is formed as the reference forms F_x[.] (mx' = rho ux times u_j), so
+// the shared off-diagonal entries differ from the reference's F_y[rho u] =
+// (rho uy) ux by one rounding at most.
+template <typename T>
+__device__ __forceinline__ T euler3_flux_sym(const T* w, int u) {
+  const T rux = w[0] * w[1], ruy = w[0] * w[2], ruz = w[0] * w[3];
+  const T f1 = u < 3 ? w[0]
+                     : ((u == 3 || u == 6 || u == 7) ? rux : ((u == 4 || u == 8) ? ruy : (u == 5 ? ruz : w[5])));
+  const T f2 = (u == 0 || u == 3 || u == 9) ? w[1] : ((u == 1 || u == 4 || u == 6 || u == 10) ? w[2] : w[3]);
+  const T v = f1 * f2;
+  return (u >= 3 && u <= 5) ? v + w[4] : v;
+}
+
 // Fluxes (or copies) of the conserved values q at the thread's x pairs ->
 // forward transform of the row -> kept modes to a.out.  Shared by k_xfwd and
 // k_xmid.
@@ -1043,7 +1093,9 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
         if (G::PALL || p < M) {
 #pragma unroll
           for (int l = 0; l < LX; ++l)
-            buf[xslot<LG>(p, g * LX + l)] = mk(euler_flux<T, NC>(pr[j][0], g, l), euler_flux<T, NC>(pr[j][1], g, l));
+            buf[xslot<LG>(p, g * LX + l)] =
+                MODE == XF_FLUX3 ? mk(euler3_flux_sym<T>(pr[j][0], g * LX + l), euler3_flux_sym<T>(pr[j][1], g * LX + l))
+                                 : mk(euler_flux<T, NC>(pr[j][0], g, l), euler_flux<T, NC>(pr[j][1], g, l));
         }
       }
     lds_barrier();
@@ -1081,6 +1133,8 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
                 v[r] = v[r] - Bg * w[4 + l];
               } else if (l == 4) v[r] = w[10] * ug - Bg * w[9];
               else v[r] = (l - 5 == g) ? T(0) : ug * w[l] - w[l - 4] * Bg;
+            } else if constexpr (MODE == XF_FLUX3) {
+              v[r] = euler3_flux_sym<T>(pr[j][r], g * LX + l);
             } else {
               v[r] = euler_flux<T, NC>(pr[j][r], g, l);
             }

@@ -352,15 +352,17 @@ struct Plan3D : PlanBase {
   CE* xcol_k = nullptr;    // kx per column of the column passes (zero past the kept modes)
   T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
   bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
+  bool symflux = true;     // Euler fluxes in the 12-field symmetric layout (x pass without i kx)
   // workspace layout (bytes)
   size_t offA, offB, offUs, offPart, offCfl, wsBytes;
   static constexpr int NPART = 1024;
 
   // na / nb: fields of the spectral buffers A (x pass output) and B (y pass
-  // output): 15 / 10 for the Euler RHS, 24 / 16 for ideal MHD
+  // output): 12 / 10 for the Euler RHS (the 12 distinct fluxes of the
+  // symmetric layout, euler3_flux_sym), 24 / 16 for ideal MHD
   Plan3D(int nx_, int ny_, int nz_, double lx_, double ly_, double lz_, bool dealias, bool filt,
-         int p, double alpha, double gamma_, int na = 15, int nb = 10)
-      : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_) {
+         int p, double alpha, double gamma_, int na = 12, int nb = 10)
+      : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_), symflux(na == 12) {
     dtype = sizeof(T) == 8 ? 0 : 1;
     if (!supported_len(nx) || !supported_len(ny) || !supported_len(nz))
       throw Error(-2, "grid sizes must be powers of two in [8, 1024]");
@@ -510,6 +512,8 @@ struct Plan3D : PlanBase {
     a.tw = dy.tw;
     a.sig = unit ? dy.one : dy.sig;
     a.kk = dy.k;
+    a.symflux = div && symflux ? 1 : 0;
+    a.colk = xcol_k;
     KScope ks(div ? "yfwd_div" : (unit ? "yfwd_unit" : "yfwd_copy"), st);
     if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
     else launch_col<CE, CM_FWD_COPY>(ny, nz, a, st);
@@ -679,7 +683,7 @@ struct Plan3D : PlanBase {
     check_launch("k_xinv");
   }
   void divergence(const T* U, void* ws, hipStream_t st) const {
-    xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st);
+    xfwd(XF_FLUX3, U, 5, 0u, ws, st);
     yfwd(true, 5, 0, ws, st);
     zpass(true, 5, 0, ws, st);
     yinv(5, ws, st);
@@ -715,7 +719,7 @@ struct Plan3D : PlanBase {
             bool bandlimited = false, const int* gate = nullptr, bool cfl = false, bool flux_in = false,
             bool flux_out = false) const {
     const bool corr = bandlimited && sizeof(T) == 4;
-    if (!flux_in) xfwd(XF_FLUX3, U, 5, 0x1Fu, ws, st, gate);
+    if (!flux_in) xfwd(XF_FLUX3, U, 5, 0u, ws, st, gate);
     for (int s = 1; s <= 4; ++s) {
       yfwd(true, 5, 0, ws, st, false, gate);
       zpass(true, 5, 0, ws, st, gate);
@@ -729,7 +733,7 @@ struct Plan3D : PlanBase {
       m.inv.dt = dt;
       m.inv.dt_dev = dt_dev;
       m.inv.keep_state = corr ? 1 : 0;
-      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws, gate);
+      m.fwd = xfwd_args(nullptr, 5, 0u, ws, gate);
       m.fwd_last = xfwd_args(nullptr, 5, 0u, ws, gate);
       if (corr) m.fwd_last.sig = dx.one;
       KScope ks(s == 4 ? (corr ? "xmid_last_keep" : "xmid_last") : (s == 1 ? "xmid1" : "xmid23"), st);
@@ -753,7 +757,7 @@ struct Plan3D : PlanBase {
     if (flux_out) {
       XMidArgs<T> m{};
       m.inv = a;
-      m.fwd = xfwd_args(nullptr, 5, 0x1Fu, ws, gate);
+      m.fwd = xfwd_args(nullptr, 5, 0u, ws, gate);
       KScope ks(corr ? "xend_add" : "xend_out", st);
       if (corr) launch_xend<T, 5, XI_ADD>(nx, ny, nz, m, st);
       else launch_xend<T, 5, XI_OUT>(nx, ny, nz, m, st);
