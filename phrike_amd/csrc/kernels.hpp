@@ -804,6 +804,10 @@ struct XMidArgs {
 #ifndef PH_XR_DIV_F64
 #define PH_XR_DIV_F64 2
 #endif
+// unroll the 3D Euler flux groups of the x forward transforms (0: a loop)
+#ifndef PH_XF_UNROLL_G
+#define PH_XF_UNROLL_G 1
+#endif
 #ifndef PH_XR_DIV_F32
 #define PH_XR_DIV_F32 4
 #endif
@@ -919,7 +923,10 @@ struct XRSpec {
     }
     constexpr int ZB = G::M + 1 - G::MX;
     if constexpr (ZB > 0) {
-      for (int t = tid; t < NCI * ZB; t += G::NT) {
+#pragma unroll
+      for (int i = 0; i < (NCI * ZB + G::NT - 1) / G::NT; ++i) {
+        const int t = tid + i * G::NT;
+        if ((NCI * ZB) % G::NT != 0 && t >= NCI * ZB) break;
         const int l = t / ZB, k = G::MX + (t - l * ZB);
         buf[xslot<NCI>(k, l)] = mk(T(0), T(0));
       }
@@ -933,7 +940,10 @@ struct XRSpec {
 template <typename T, int NX, int LINES, int NT>
 __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
   constexpr int M = NX / 2, H = M / 2 + 1;
-  for (int t = tid; t < LINES * H; t += NT) {
+#pragma unroll
+  for (int i = 0; i < (LINES * H + NT - 1) / NT; ++i) {
+    const int t = tid + i * NT;
+    if ((LINES * H) % NT != 0 && t >= LINES * H) break;
     const int l = t / H, k = t - l * H, k2 = M - k;
     cx<T> a = buf[xslot<LINES>(k, l)], b = buf[xslot<LINES>(k2, l)];
     if (k == 0) {  // X[0], X[M]: real parts
@@ -955,7 +965,7 @@ __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
 // X[k] = (E + W^k O) * sig[k] (* i k[k] where bit (l / GL) * GL of ikmask is
 // set) of lines l < nl, stored as 16-byte chunks to dst (one row of the
 // line's field; field stride fs)
-template <typename T, int NX, bool DA, int LX, int GL = LX>
+template <typename T, int NX, bool DA, int LX, int GL = LX, bool IK = true>
 __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw, cx<T>* dst, long long fs, int nl,
                                                const T* sig, const T* kk, unsigned ikmask, int tid) {
   using G = XRGeo<T, NX, DA>;
@@ -963,9 +973,11 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
   constexpr int NTASK = LX * G::MCP, M = G::M;
   C* d = reinterpret_cast<C*>(dst);
   const long long fsc = fs / G::K;
-  for (int t = tid; t < NTASK; t += G::NT) {
+#pragma unroll
+  for (int i = 0; i < (NTASK + G::NT - 1) / G::NT; ++i) {
+    const int t = tid + i * G::NT;
     const int l = t / G::MCP, m = t - l * G::MCP;
-    if (l < nl) {
+    if ((NTASK % G::NT == 0 || t < NTASK) && l < nl) {
       C c{};
 #pragma unroll
       for (int i = 0; i < G::K; ++i) {
@@ -977,7 +989,7 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
           const cx<T> h = mk(T(0.5) * (a.x - z2.x), T(0.5) * (a.y + z2.y));  // (Z[k] - Z*[M-k]) / 2
           cx<T> X = e + cmul(tw[k], mk(h.y, -h.x));                           // E + W^k (-i h)
           X = cscale(X, sig[k]);
-          if ((ikmask >> ((l / GL) * GL)) & 1u) X = mul_ik(X, kk[k]);  // i k of line l's group
+          if (IK && ((ikmask >> ((l / GL) * GL)) & 1u)) X = mul_ik(X, kk[k]);  // i k of line l's group
           ch_set(c, i, X);
         }
       }
@@ -1100,12 +1112,17 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
       }
     lds_barrier();
     fft_lds<T, M, LG, -1, NT, xstride(LG), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
-    xr_split_store<T, NX, DA, LG, LX>(buf, tw, dst, a.out_fs, LG, sig, kk, a.ikmask, tid);
+    xr_split_store<T, NX, DA, LG, LX, MODE != XF_FLUX3>(buf, tw, dst, a.out_fs, LG, sig, kk, a.ikmask, tid);
     lds_barrier();
     return;
   }
   const int nl = xf_copy(MODE) ? ncomp : LX;
-#pragma unroll 1
+  // the 3D Euler groups unrolled: each line's flux is then fixed at compile
+  // time (a run-time group index selects it through chains of selects)
+  constexpr int GU = (MODE == XF_FLUX3 && PH_XF_UNROLL_G) ? S::NG : 1;
+  // the 3D Euler fluxes are stored without i k (the y pass applies i kx)
+  constexpr bool IK = MODE != XF_FLUX3;
+#pragma unroll GU
   for (int g = 0; g < S::NG; ++g) {
 #pragma unroll
     for (int j = 0; j < PJ; ++j) {
@@ -1145,8 +1162,8 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
     }
     lds_barrier();
     fft_lds<T, M, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
-    xr_split_store<T, NX, DA, LX>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
-                                  (a.ikmask >> (g * LX)) & 1u, tid);
+    xr_split_store<T, NX, DA, LX, LX, IK>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
+                                          (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();
   }
 }
