@@ -74,7 +74,7 @@ def kernel_symbol(tag: str, n: int, prec: str) -> str:
             "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid_last_keep": f"k_xmid<{T}, {n}, true, 5, 0>",
             "yfwd_unit": f"k_col<{C}, {n}, 1>", "z_corr": f"k_col<{C}, {n}, 5>",
             "xinv_add": f"k_xinv<{T}, {n}, true, 5, 4>",
-            "xend_out": f"k_xend<{T}, {n}, true, 0>", "xend_add": f"k_xend<{T}, {n}, true, 4>"}.get(tag, tag)
+            "xend_out": f"k_xend<{T}, {n}, true, 5, 0>", "xend_add": f"k_xend<{T}, {n}, true, 5, 4>"}.get(tag, tag)
 
 
 # launches of each timing tag in one RK4 step of the fused 3D pipeline, in

@@ -58,16 +58,30 @@ __device__ __forceinline__ double rcp_or_one(double v) { return v == 0.0 ? 1.0 :
 __device__ __forceinline__ float rcp_or_one(float v) { return v == 0.f ? 1.f : 1.f / v; }
 __device__ __forceinline__ f2 rcp_or_one(f2 v) { return f2(rcp_or_one(v.a), rcp_or_one(v.b)); }
 
-// Spectral field holding the 3D Euler flux F_d[c] in the x pass output of
-// the symmetric layout (the 12 distinct fluxes, euler3_flux_sym; d = 0, 1, 2
-// for x, y, z; c = 0..4), 4 bits per (c, d) at bit 4 (3 c + d)
-constexpr unsigned long long kEulerSymField =
-    (0ull << 0) | (1ull << 4) | (2ull << 8) |        // rho:  u0 u1 u2
-    (3ull << 12) | (6ull << 16) | (7ull << 20) |     // rho u: u3 u6 u7
-    (6ull << 24) | (4ull << 28) | (8ull << 32) |     // rho v: u6 u4 u8
-    (7ull << 36) | (8ull << 40) | (5ull << 44) |     // rho w: u7 u8 u5
-    (9ull << 48) | (10ull << 52) | (11ull << 56);    // E:     u9 u10 u11
-__device__ __forceinline__ int sym_field(int c, int d) { return int((kEulerSymField >> (4 * (3 * c + d))) & 15ull); }
+// The y pass's order of the 12 distinct 3D Euler fluxes of the symmetric
+// layout (euler3_flux_sym's u), each transformed once, 4 bits per step:
+// mass (x, y, z), rho ux uy, xx, yy, xz, yz, zz, energy (x, y, z)
+constexpr unsigned long long kEulerSymSeq =
+    (0ull << 0) | (1ull << 4) | (2ull << 8) |                     // u0 u1 u2
+    (6ull << 12) | (3ull << 16) | (4ull << 20) |                  // u6 u3 u4
+    (7ull << 24) | (8ull << 28) | (5ull << 32) |                  // u7 u8 u5
+    (9ull << 36) | (10ull << 40) | (11ull << 44);                 // u9 u10 u11
+__device__ __forceinline__ int sym_seq(int k) { return int((kEulerSymSeq >> (4 * k)) & 15ull); }
+// per step of that order: the output field of sigma F(Fz[c]) (5 + c) and of
+// G[c] (c), 15 for none; whether the step's spectrum is kept for the next G,
+// and whether the kept spectrum is that G's x flux (else its y flux)
+constexpr unsigned long long kEulerSymZ =
+    (15ull << 0) | (15ull << 4) | (5ull << 8) | (15ull << 12) | (15ull << 16) | (15ull << 20) |
+    (6ull << 24) | (7ull << 28) | (8ull << 32) | (15ull << 36) | (15ull << 40) | (9ull << 44);
+constexpr unsigned long long kEulerSymG =
+    (15ull << 0) | (0ull << 4) | (15ull << 8) | (15ull << 12) | (1ull << 16) | (2ull << 20) |
+    (15ull << 24) | (3ull << 28) | (15ull << 32) | (15ull << 36) | (4ull << 40) | (15ull << 44);
+constexpr unsigned kEulerSymKeep = (1u << 0) | (1u << 3) | (1u << 6) | (1u << 9);
+constexpr unsigned kEulerSymFirstX = (1u << 1) | (1u << 5) | (1u << 7) | (1u << 10);
+__device__ __forceinline__ int sym_z(int k) { return int((kEulerSymZ >> (4 * k)) & 15ull); }
+__device__ __forceinline__ int sym_g(int k) { return int((kEulerSymG >> (4 * k)) & 15ull); }
+__device__ __forceinline__ bool sym_keep(int k) { return (kEulerSymKeep >> k) & 1u; }
+__device__ __forceinline__ bool sym_first_x(int k) { return (kEulerSymFirstX >> k) & 1u; }
 
 // ----------------------------------------------------------------------------
 // retained-mode bookkeeping along one axis: positions n <= kp and
@@ -294,8 +308,9 @@ struct ColArgs {
   // blockIdx.y (by blockIdx.y); this axis's k is kk
   const T* colk;
   const T* rowk;
-  // CM_FWD_DIV3 with the 12-field symmetric 3D Euler flux layout
-  // (sym_field): F_x[c] is read without i kx, which is applied here (colk)
+  // CM_FWD_DIV3 with the 12-field symmetric 3D Euler flux layout (nc = 5;
+  // order kEulerSymSeq): F_x[c] is read without i kx, which is applied here
+  // (colk)
   int symflux;
 };
 
@@ -412,26 +427,21 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   }
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
-    if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_field(k / 3, k % 3) : (k % 3) * nc + k / 3;
+    if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
     else if constexpr (MODE == CM_FWDINV_DIV) return (k % 2) * nc + k / 2;
     else return k;
   };
   constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : (MODE == CM_FWDINV_DIV ? 2 : 1);
-  const int nsrc = PERC * nc;
+  const int nsrc = (MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * nc;
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
-    const int f = field_of(k);
-    const cx<T>* src = in + (long long)f * a.in_fs + c0;
-    // the symmetric layout's shared fluxes (fields 6-8) are read twice per
-    // tile: cached loads, so the second read can come from the caches
-    const bool shared = MODE == CM_FWD_DIV3 && a.symflux && f >= 6 && f <= 8;
+    const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
     return [=](int l, int n) {
       if constexpr (MODE == CM_INV)
         return (c0 + l < ncol && kp.kept(n, N)) ? ld(src + (long long)kp.compact(n, N) * ls + l)
                                                 : mk(T(0), T(0));
       else
-        return (c0 + l < ncol) ? (shared ? src[(long long)n * ls + l] : ld(src + (long long)n * ls + l))
-                               : mk(T(0), T(0));
+        return (c0 + l < ncol) ? ld(src + (long long)n * ls + l) : mk(T(0), T(0));
     };
   };
   // visit every (q, r) output slot of this thread: f(q, r, l, n, col_ok)
@@ -483,17 +493,50 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         }
       });
     } else if constexpr (MODE == CM_FWD_DIV3) {
+      if (a.symflux) {
+        // 3D Euler, symmetric layout (nc = 5; no i kx in the x pass): the 12
+        // distinct fluxes are transformed once each, in the order sym_seq,
+        // and every spectrum serves all the outputs that use it:
+        // G[c] = sigma (i kx F(Fx[c]) + i ky F(Fy[c])) -> field c (sym_g),
+        // sigma F(Fz[c]) -> field 5 + c (sym_z).  G is formed with the kept
+        // spectrum (first: the x flux where sym_first_x, else the y flux).
+#pragma unroll 1
+        for (int s = 0; s < 12; ++s) {
+          fwd(FWD{});
+          const int zf = sym_z(s), gf = sym_g(s);
+          const bool fix = sym_first_x(s), keep = sym_keep(s);
+          cx<T>* oz = out + (long long)zf * a.out_fs + c0;
+          cx<T>* og = out + (long long)gf * a.out_fs + c0;
+          each([&](int q, int r, int l, int n, bool ok) {
+            const cx<T> x = X[q][r];
+            if (ok && kp.kept(n, N)) {
+              const int j = kp.compact(n, N);
+              const T sg = a.sig[j];
+              if (zf < 15) st(oz + (long long)j * ols + l, cscale(x, sg));
+              if (gf < 15) {
+                const cx<T> y0 = first(q, r, l, n);
+                const cx<T> fx = fix ? y0 : x;
+                const cx<T> fy = fix ? x : y0;
+                st(og + (long long)j * ols + l, cscale(mul_ik(fx, s_colk[l]) + mul_ik(fy, a.kk[j]), sg));
+              }
+            }
+            if (keep) {
+              if constexpr (YREG) Y[q][r] = x;
+              else if (kp.kept(n, N)) ybuf[n * TM + l] = x;
+            }
+          });
+        }
+        break;
+      }
       // G = sigma (X + i k Z) with X = F(i kx Fx^), Z = F(Fy^); then sigma
-      // F(Fz^).  The symmetric layout stores Fx^ without i kx: applied here.
+      // F(Fz^)
       fwd(FWD{});
       keep_first();
       fwd(FWD{});
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok && kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          const cx<T> x0 = first(q, r, l, n);
-          const cx<T> x = a.symflux ? mul_ik(x0, s_colk[l]) : x0;
-          st(oc + (long long)j * ols + l, cscale(x + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
+          st(oc + (long long)j * ols + l, cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]));
         }
       });
       cx<T>* o2 = out + (long long)(nc + c) * a.out_fs + c0;

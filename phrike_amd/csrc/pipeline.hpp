@@ -513,6 +513,7 @@ struct Plan3D : PlanBase {
     a.sig = unit ? dy.one : dy.sig;
     a.kk = dy.k;
     a.symflux = div && symflux ? 1 : 0;
+    if (a.symflux && nc != 5) throw Error(-2, "the symmetric flux layout takes the 5 Euler fields");
     a.colk = xcol_k;
     KScope ks(div ? "yfwd_div" : (unit ? "yfwd_unit" : "yfwd_copy"), st);
     if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
