@@ -35,6 +35,7 @@ def byte_model(n: int):
     k = n // 3
     a = 2 * (k + 1) / n
     b = 2 * (k + 1) * (2 * k + 1) / n ** 2
+    c = b * (2 * k + 1) / n  # one field's kept modes of all three axes
     # the x kernels write the 12 distinct Euler fluxes (the momentum flux
     # tensor is symmetric), which the y pass reads
     per = {
@@ -53,6 +54,11 @@ def byte_model(n: int):
         # chained steps: the filter's x inverse also forms the next step's
         # fluxes and their x transform (the state is not read again)
         "xend_out": 5 * a + 5 + 12 * a, "xend_add": 5 * a + 5 + 5 + 12 * a,
+        # the RK4 sum in spectral space: stage boundaries without an
+        # accumulator; z passes that also read and write the sum S (stage 4:
+        # the filtered new state's spectrum); an unchained step's first pass
+        # of S (after the x forward and unit-factor y pass of the state)
+        "xmid_sa": 5 * a + 5 + 12 * a, "z_acc": 15 * b + 10 * c, "z_fwd": 5 * b + 5 * c,
     }
     # per-step model of the unfused pencil pipeline (stage states written and
     # re-read), the official figure for hbm_frac_step
@@ -74,26 +80,33 @@ def kernel_symbol(tag: str, n: int, prec: str) -> str:
             "xmid_last": f"k_xmid<{T}, {n}, true, 5, 0>", "xmid_last_keep": f"k_xmid<{T}, {n}, true, 5, 0>",
             "yfwd_unit": f"k_col<{C}, {n}, 1>", "z_corr": f"k_col<{C}, {n}, 5>",
             "xinv_add": f"k_xinv<{T}, {n}, true, 5, 4>",
-            "xend_out": f"k_xend<{T}, {n}, true, 5, 0>", "xend_add": f"k_xend<{T}, {n}, true, 5, 4>"}.get(tag, tag)
+            "xend_out": f"k_xend<{T}, {n}, true, 5, 0>", "xend_add": f"k_xend<{T}, {n}, true, 5, 4>",
+            "xmid_sa": f"k_xmid<{T}, {n}, true, 5, 0>", "z_acc": f"k_col<{C}, {n}, 7>",
+            "z_fwd": f"k_col<{C}, {n}, 1>"}.get(tag, tag)
 
 
 # launches of each timing tag in one RK4 step of the fused 3D pipeline, in
-# a chain of steps (each step's last kernel forms the next step's first x pass)
-STEP_LAUNCHES = {"yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
-                 "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xend_out": 1}
-# FP32 steps on a previous step's output (the filter in correction form)
+# a chain of steps (each step's last kernel forms the next step's first x
+# pass, and leaves the filtered state's spectrum for the next step's RK4 sum)
+STEP_LAUNCHES = {"yfwd_div": 4, "z_acc": 4, "yinv": 4, "xmid_sa": 3, "xend_out": 1}
+# the library built with the RK4 accumulator in the x kernels (PH_SPECACC=0):
+# FP64, and FP32 steps on a previous step's output (filter in correction form)
+STEP_LAUNCHES_XACC = {"yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
+                      "xmid_last": 1, "yfwd_copy": 1, "z_copy": 1, "xend_out": 1}
 STEP_LAUNCHES_CORR = {"yfwd_div": 4, "z_div": 4, "yinv": 5, "xmid1": 1, "xmid23": 2,
                       "xmid_last_keep": 1, "yfwd_unit": 1, "z_corr": 1, "xend_add": 1}
 
 
-def step_launches(prec: str):
-    return STEP_LAUNCHES_CORR if prec == "single" else STEP_LAUNCHES
+def step_launches(prec: str, specacc: bool = True):
+    if specacc:
+        return STEP_LAUNCHES
+    return STEP_LAUNCHES_CORR if prec == "single" else STEP_LAUNCHES_XACC
 
 
-def fused_step_units(n: int, prec: str = "double") -> float:
+def fused_step_units(n: int, prec: str = "double", specacc: bool = True) -> float:
     """Bytes per step (units of s*N^3) that the fused pipeline moves."""
     per, _ = byte_model(n)
-    return sum(per[t] * k for t, k in step_launches(prec).items())
+    return sum(per[t] * k for t, k in step_launches(prec, specacc).items())
 
 
 def cpu_baseline(n: int, steps: int):
@@ -847,6 +860,9 @@ def main():
         kt = _lib.timing_collect()
     ms_per_step = ev_ms / args.steps
     value = n ** 3 * args.steps / (ev_ms / 1e3) / 1e6
+    # the step keeps its RK4 sum in spectral space unless the library was
+    # built without it (its timing tags say which)
+    specacc = not kt or "xmid_sa" in kt
     es = 8 if args.precision == "double" else 4
     per, C = byte_model(n)
     step_bytes = C * es * n ** 3
@@ -950,9 +966,9 @@ def main():
                    "grid": [n, n, n], "parallelism": "single GPU"},
         "hbm_frac_step": round(step_bytes / (ms_per_step / 1e3) / HBM_PEAK, 4),
         "step_model_bytes": int(step_bytes),
-        "step_fused_bytes": int(fused_step_units(n, args.precision) * es * n ** 3),
-        "hbm_frac_step_fused": round(fused_step_units(n, args.precision) * es * n ** 3 / (ms_per_step / 1e3)
-                                     / HBM_PEAK, 4),
+        "step_fused_bytes": int(fused_step_units(n, args.precision, specacc) * es * n ** 3),
+        "hbm_frac_step_fused": round(fused_step_units(n, args.precision, specacc) * es * n ** 3
+                                     / (ms_per_step / 1e3) / HBM_PEAK, 4),
         # the dense (unpruned) pencil model, C = 420, for transparency
         "hbm_frac_step_dense": round(420 * es * n ** 3 / (ms_per_step / 1e3) / HBM_PEAK, 4),
         "wall_s": round(wall, 3),

@@ -92,24 +92,35 @@ int ph_deriv3d(ph_plan3d_t plan, int axis, int nfields, const void* f, void* out
 int ph_step3d(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev,
               void* ws, void* stream);
 
-/* ph_step3d with flags.  PH_STEP_BANDLIMITED: U has no content beyond the 2/3
- * cutoff (true of every output of a previous step with the mask on, whose RHS
- * and filter are both masked).  FP32 plans then apply the post-step filter in
- * correction form, out = U' + F^-1((sx sy sz - 1) F U') on the kept modes,
- * which is the same operator on such states; the FP32 transforms then carry
- * only the small correction instead of the whole state.  FP64 plans ignore
- * the flag (direct form, the reference's operation order).  No reference
- * counterpart: a precision choice of this backend's FP32 plans. */
+/* ph_step3d with flags.
+ *
+ * With the 2/3 rule on (every workload), a 3D Euler step keeps its RK4 sum in
+ * spectral space: it starts from S = the masked spectrum of U, each stage
+ * subtracts w D (the stage's RHS spectrum is -D; w = dt/6, dt/3, dt/3, dt/6)
+ * and stage 4 leaves sigma_x sigma_y sigma_z S, the spectrum of the filtered
+ * new state, which is transformed back.  That is the reference's post-step
+ * filter of U + dt/6 (k1 + 2 k2 + 2 k3 + k4) (phrike/solver.py:1220-1234),
+ * since each k is band-limited, to rounding.
+ *
+ * PH_STEP_BANDLIMITED: U has no content beyond the 2/3 cutoff (true of every
+ * output of a previous step).  Plans without the 2/3 rule, and a library
+ * built with PH_SPECACC=0, then apply the FP32 post-step filter in
+ * correction form, out = U' + F^-1((sx sy sz - 1) F U') on the kept modes
+ * (the FP32 transforms carry only the small correction); other plans ignore
+ * the flag.  No reference counterpart: a precision choice. */
 #define PH_STEP_BANDLIMITED 1
 /* PH_STEP_CHAIN: end the step with the x transform of its output's fluxes
  * (the next step's first pass, formed while the output is in registers) and
- * keep them in the workspace.  PH_STEP_CHAINED: U is the unmodified output of
- * a PH_STEP_CHAIN step of this plan; if the workspace still holds U's fluxes
- * the step skips its own first pass.  That is checked by address (the plan,
- * the workspace, U equal to that step's output, and no call since that wrote
- * the workspace's spectral buffers); otherwise the step computes them.  The
- * caller promises only that U's contents are unchanged.  No reference
- * counterpart: a fusion across the reference's step boundary. */
+ * keep them, and the output's spectrum S, in the workspace.
+ * PH_STEP_CHAINED: U is the unmodified output of a PH_STEP_CHAIN step of this
+ * plan (or the final state of a PH_STEP_CHAIN ph_run3d call); if the
+ * workspace still holds U's fluxes and spectrum the step skips its own first
+ * passes (else it transforms U itself; the two agree to rounding).  That is
+ * checked by address (the plan, the workspace, U equal to that step's output
+ * or one of that call's buffers, and no call since that wrote the
+ * workspace's spectral buffers).  The caller promises that U's contents are
+ * unchanged and, after ph_run3d, that U is the buffer holding its final
+ * state.  No reference counterpart: a fusion across the step boundary. */
 #define PH_STEP_CHAIN 4
 #define PH_STEP_CHAINED 8
 int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const double* dt_dev, int flags,
@@ -132,7 +143,10 @@ int ph_step3d_ex(ph_plan3d_t plan, const void* U, void* out, double dt, const do
  * synchronises the host, which reads t, steps and active back only when it
  * wants the state (outputs, checkpoints, the end).  `flags` (ph_run3d):
  * PH_STEP_BANDLIMITED for the first step when Ua is a previous step's
- * output; later steps always are.  ph_run2d applies the post-step filter
+ * output (later steps always are); the steps of one call are chained;
+ * PH_STEP_CHAIN also chains the call's last step to the next call (which
+ * passes PH_STEP_CHAINED with the buffer holding this call's final state as
+ * its Ua), so that a run in several calls computes what one call would.  ph_run2d applies the post-step filter
  * every step (filter interval 1).  Replaces the run loops of
  * SpectralSolver3D.run (phrike/solver.py:1475-1484) and SpectralSolver2D.run
  * (phrike/solver.py:764-779). */

@@ -214,13 +214,18 @@ int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, i
   return guard([&] {
     need(Ua && Ub && rs_dev && rf_dev && ws, "null buffer");
     need(nsteps >= 1, "nsteps must be >= 1");
-    need((flags & ~PH_STEP_BANDLIMITED) == 0, "unknown step flags");
-    ph::chain_registry().forget(ws);
+    need((flags & ~(PH_STEP_BANDLIMITED | PH_STEP_CHAIN | PH_STEP_CHAINED)) == 0, "unknown step flags");
     with3(plan, [&](auto& p) {
       using R = std::conditional_t<std::is_same_v<std::decay_t<decltype(p)>, ph::Plan3D<double>>, double, float>;
       const size_t n = 5ull * size_t(p.ncell) * sizeof(R);
       need(!ph::ranges_overlap(Ua, n, Ub, n), "Ua and Ub must not overlap");
       hipStream_t st = S(stream);
+      auto& reg = ph::chain_registry();
+      const bool chain = (flags & PH_STEP_CHAIN) != 0;
+      // PH_STEP_CHAINED: Ua is the state a previous chained call on this
+      // workspace left, and nothing has written the workspace since
+      const bool chained_in = (flags & PH_STEP_CHAINED) != 0 && reg.holds(ws, &p, Ua);
+      reg.forget(ws);
       for (int k = 0; k < nsteps; ++k) {
         R* src = static_cast<R*>(k % 2 ? Ub : Ua);
         R* dst = static_cast<R*>(k % 2 ? Ua : Ub);
@@ -232,12 +237,17 @@ int ph_run3d(ph_plan3d_t plan, void* Ua, void* Ub, int nsteps, double* rs_dev, i
         // steps chain within the call: each but the last leaves the fluxes of
         // its output for the next (a step of a stopped run does nothing, and
         // so do all after it)
+        // with PH_STEP_CHAIN the last step also leaves them for the next
+        // call, and a first step that computes its own does so even if the
+        // run has stopped, so the workspace holds those of the call's final
+        // state (Ua or Ub, by the number of steps taken) in every case
         p.step(src, dst, 0.0, rs_dev + ph::RS_DT, ws, st, k > 0 || (flags & PH_STEP_BANDLIMITED) != 0, rf_dev,
-               true, k > 0, k + 1 < nsteps);
+               true, k > 0 || chained_in, k + 1 < nsteps || chain, k == 0 && chain);
         p.max_rows(nullptr, rs_dev + ph::RS_DTNEXT, cfl, ws, st, rf_dev);
         hipLaunchKernelGGL(ph::k_run_advance, dim3(1), dim3(ph::NT), 0, st, 1, rs_dev, rf_dev);
         ph::check_launch("k_run_advance");
       }
+      if (chain) reg.set(ws, &p, Ua, Ub);
     });
   });
 }

@@ -272,14 +272,16 @@ struct XMerge {
 // transform axis with element stride ls.
 // ----------------------------------------------------------------------------
 enum ColMode {
-  CM_FWD_DIV3 = 0,    // 3D y forward: A(15) -> B(10), compact out
+  CM_FWD_DIV3 = 0,    // 3D y forward: A(12 or 15) -> B(10), compact out
   CM_FWD_COPY = 1,    // forward, sigma (and i*k), compact out, nc fields
   CM_FWDINV_DIV = 2,  // forward both halves, D = sigma (X + i k Z), inverse, full out
   CM_FWDINV_COPY = 3, // forward, sigma (and i*k) on kept modes, inverse, full out
   CM_INV = 4,         // compact in, zero-filled inverse, full out
   CM_FWDINV_CORR = 5, // forward, (sx sy sz - 1) on kept modes, inverse, full out
-  CM_FWDINV_PROJ = 6  // three fields (B) together: forward, sigma and the Leray
+  CM_FWDINV_PROJ = 6, // three fields (B) together: forward, sigma and the Leray
                       // projection on kept modes, inverse, full out
+  CM_FWDINV_DIVACC = 7  // CM_FWDINV_DIV and the RK4 sum of the step's RHS spectra
+                        // (below)
 };
 
 template <typename T>
@@ -308,6 +310,21 @@ struct ColArgs {
   // blockIdx.y (by blockIdx.y); this axis's k is kk
   const T* colk;
   const T* rowk;
+  // CM_FWDINV_DIVACC: the step's RK4 sum in 3D spectral space, S (kept
+  // modes of all three axes; element (field c, kept z j, blockIdx.y, column)
+  // at acc + c acc_fs + j acc_ls + blockIdx.y acc_bs + column).  With D this
+  // stage's RHS spectrum (-D the transform of k), stages 1-3 leave
+  // S - w D in S (stage 1 starts from S = F U, the input state's masked
+  // spectrum) and transform D back; stage 4 leaves U' = sigma_x sigma_y
+  // sigma_z (S - dt/6 D), the filtered new state's spectrum, in S and
+  // transforms U' back.  w = dt/6, dt/3, dt/3, dt/6; dt = dt_dev[0] or dt.
+  cx<T>* acc;
+  long long acc_fs, acc_bs, acc_ls;
+  int stage;
+  double dt;
+  const double* dt_dev;
+  const T* colsig;  // sigma of the column axis (by column)
+  const T* rowsig;  // sigma of the axis of blockIdx.y (by blockIdx.y)
   // CM_FWD_DIV3 with the 12-field symmetric 3D Euler flux layout (nc = 5;
   // order kEulerSymSeq): F_x[c] is read without i kx, which is applied here
   // (colk)
@@ -350,7 +367,7 @@ struct ColArgs {
 
 template <typename T, int N, int MODE>
 struct ColTile {
-  static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV);
+  static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC);
   static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
   static constexpr int ELEMS = MODE == CM_FWDINV_PROJ
                                    ? PH_COL_ELEMS_PROJ
@@ -428,10 +445,10 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
     if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
-    else if constexpr (MODE == CM_FWDINV_DIV) return (k % 2) * nc + k / 2;
+    else if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) return (k % 2) * nc + k / 2;
     else return k;
   };
-  constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : (MODE == CM_FWDINV_DIV ? 2 : 1);
+  constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : ((MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) ? 2 : 1);
   const int nsrc = (MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * nc;
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
@@ -595,10 +612,11 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       }
       break;
     } else {
-      // CM_FWDINV_DIV / CM_FWDINV_COPY / CM_FWDINV_CORR: forward, combine on
-      // kept modes, inverse
+      // CM_FWDINV_DIV / CM_FWDINV_DIVACC / CM_FWDINV_COPY / CM_FWDINV_CORR:
+      // forward, combine on kept modes, inverse
+      constexpr bool DIV = MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC;
       fwd(FWD{});
-      if constexpr (MODE == CM_FWDINV_DIV) {
+      if constexpr (DIV) {
         keep_first();
         fwd(FWD{});
       }
@@ -606,7 +624,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         cx<T> v = mk(T(0), T(0));
         if (kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          if constexpr (MODE == CM_FWDINV_DIV) {
+          if constexpr (DIV) {
             v = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
           } else if constexpr (MODE == CM_FWDINV_CORR) {
             // sx sy sz - 1 from the three (sigma - 1): no cancellation
@@ -621,6 +639,35 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         buf[n * TM + l] = v;
       });
       lds_barrier();
+      if constexpr (MODE == CM_FWDINV_DIVACC) {
+        // the RK4 sum on the tile's kept modes, D read back from buf in
+        // whole tile rows (X is free here, so the S loads of a thread are
+        // issued together)
+        const double dtd = a.dt_dev != nullptr ? a.dt_dev[0] : a.dt;
+        const bool last = a.stage >= 4;
+        const T w = T(dtd * ((a.stage == 1 || last) ? 1.0 / 6.0 : 1.0 / 3.0));
+        const T sy = a.rowsig[blockIdx.y];
+        cx<T>* sc = a.acc + (long long)c * a.acc_fs + (long long)blockIdx.y * a.acc_bs + c0;
+        constexpr int NTASK = ((N / 3) * 2 + 1) * TM;  // kept positions at most (2/3 rule)
+#pragma unroll
+        for (int i = 0; i < (NTASK + NT - 1) / NT; ++i) {
+          const int t = tid + i * NT;
+          const int j = t / TM, l = t - j * TM;
+          if (j < kp.nr && c0 + l < ncolw) {
+            const int n = kp.natural(j, N);
+            cx<T>* sp = sc + (long long)j * a.acc_ls + l;
+            const cx<T> s1 = ld(sp) - cscale(buf[n * TM + l], w);
+            if (last) {
+              const cx<T> u = cscale(s1, (a.colsig[c0 + l] * sy) * a.sig[j]);
+              st(sp, u);
+              buf[n * TM + l] = u;
+            } else {
+              st(sp, s1);
+            }
+          }
+        }
+        if (last) lds_barrier();
+      }
       fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
       each([&](int q, int r, int l, int n, bool ok) {
         if (ok) st(oc + (long long)n * ols + l, X[q][r]);
@@ -659,6 +706,8 @@ struct XInvArgs {
   const double* dt_dev;    // device dt: dt_dev[slab] (dt_per_slab) or dt_dev[0]; or null
   int dt_per_slab;         // 1: one dt per slab (batched 2D runs)
   int keep_state;          // k_xmid stage 4: also write the new state to out
+  int specacc;             // k_xmid stages 1-3: no accumulator (the RK4 sum is
+                           // kept in spectral space by the z pass)
   const int* active;       // per-slab flag (batched 2D) or null
   int act_all;             // 1: active[0] gates every slab
   T gamma, rho_min, p_min;
@@ -1535,7 +1584,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       }
     }
     if constexpr (PRE_ACC) {
-      if (a.stage > 1) {
+      if (a.stage > 1 && !a.specacc) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
       }
@@ -1549,7 +1598,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       }
     }
     if constexpr (!PRE_ACC) {
-      if (a.stage > 1) {
+      if (a.stage > 1 && !a.specacc) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) xr_load_pairs<G>(row(a.out, c), tid, acc[c]);
       }
@@ -1583,7 +1632,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
     }
     // keep_state (stage 4 also writes the new state) is an option of the
     // unclamped 3D Euler form only, where the stage-4 accumulator is the state
-    if (a.stage < 4 || (NC == 5 && a.keep_state)) {
+    if (!a.specacc && (a.stage < 4 || (NC == 5 && a.keep_state))) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) xr_store_pairs<G>(row(a.out, c), tid, an[c]);
     }

@@ -15,6 +15,11 @@
 #ifndef PH_B_MYZ
 #define PH_B_MYZ 0
 #endif
+// 3D Euler steps keep the RK4 sum in spectral space (0: in the state's
+// layout, in the x kernels, with the post-step filter as separate passes)
+#ifndef PH_SPECACC
+#define PH_SPECACC 1
+#endif
 
 namespace ph {
 
@@ -238,11 +243,16 @@ inline void check_launch(const char* what) { hip_check(hipGetLastError(), what);
 // here, by workspace address: a chained step records (plan, state), every
 // other operation that writes a workspace's spectral buffers forgets it.
 struct ChainRegistry {
+  struct Entry {
+    const void* plan;
+    const void* state;
+    const void* alt;  // a run loop's other buffer (its step count picks the state), or null
+  };
   std::mutex mu;
-  std::map<const void*, std::pair<const void*, const void*>> held;  // ws -> (plan, state)
-  void set(const void* ws, const void* plan, const void* state) {
+  std::map<const void*, Entry> held;  // ws -> entry
+  void set(const void* ws, const void* plan, const void* state, const void* alt = nullptr) {
     std::lock_guard<std::mutex> g(mu);
-    held[ws] = {plan, state};
+    held[ws] = {plan, state, alt};
   }
   void forget(const void* ws) {
     std::lock_guard<std::mutex> g(mu);
@@ -251,7 +261,8 @@ struct ChainRegistry {
   bool holds(const void* ws, const void* plan, const void* state) {
     std::lock_guard<std::mutex> g(mu);
     auto it = held.find(ws);
-    return it != held.end() && it->second.first == plan && it->second.second == state;
+    return it != held.end() && it->second.plan == plan &&
+           (it->second.state == state || (it->second.alt != nullptr && it->second.alt == state));
   }
 };
 inline ChainRegistry& chain_registry() {
@@ -353,8 +364,13 @@ struct Plan3D : PlanBase {
   T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
   bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
   bool symflux = true;     // Euler fluxes in the 12-field symmetric layout (x pass without i kx)
+  // the Euler step keeps its RK4 sum in 3D spectral space (S, updated by the
+  // z passes; the 2/3 rule on): no accumulator in the x kernels, and the
+  // post-step filter is the stage-4 z pass's factor on S
+  bool specacc = false;
+  CE* xcol_sig = nullptr;  // sigma_x per column of the column passes (zero past the kept modes)
   // workspace layout (bytes)
-  size_t offA, offB, offUs, offPart, offCfl, wsBytes;
+  size_t offA, offB, offUs, offS, offPart, offCfl, wsBytes;
   static constexpr int NPART = 1024;
 
   // na / nb: fields of the spectral buffers A (x pass output) and B (y pass
@@ -362,7 +378,8 @@ struct Plan3D : PlanBase {
   // symmetric layout, euler3_flux_sym), 24 / 16 for ideal MHD
   Plan3D(int nx_, int ny_, int nz_, double lx_, double ly_, double lz_, bool dealias, bool filt,
          int p, double alpha, double gamma_, int na = 12, int nb = 10)
-      : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_), symflux(na == 12) {
+      : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_), symflux(na == 12),
+        specacc(na == 12 && dealias && PH_SPECACC != 0) {
     dtype = sizeof(T) == 8 ? 0 : 1;
     if (!supported_len(nx) || !supported_len(ny) || !supported_len(nz))
       throw Error(-2, "grid sizes must be powers of two in [8, 1024]");
@@ -402,6 +419,16 @@ struct Plan3D : PlanBase {
       }
       hip_check(hipMalloc(&xcol_k, sizeof(CE) * ncolw), "hipMalloc(column k)");
       hip_check(hipMemcpy(xcol_k, hk.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column k");
+      std::vector<CE> hs(ncolw, CE(0.0));
+      for (int c = 0; c < ncol; ++c) {
+        if constexpr (PW == 2) {
+          hs[c] = CE(float(ax.sig[2 * c]), float(2 * c + 1 < mx ? ax.sig[2 * c + 1] : 0.0));
+        } else {
+          hs[c] = CE(ax.sig[c]);
+        }
+      }
+      hip_check(hipMalloc(&xcol_sig, sizeof(CE) * ncolw), "hipMalloc(column sigma)");
+      hip_check(hipMemcpy(xcol_sig, hs.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column sigma");
     }
     {
       std::vector<T> ones(mx, T(1));
@@ -416,13 +443,16 @@ struct Plan3D : PlanBase {
     offA = 0;
     offB = al(offA + szA);
     offUs = offB;  // unused: the fused step keeps stage states in registers
-    offPart = al(offB + szB);
+    const size_t szS = specacc ? (size_t)5 * az.nr * ay.nr * mxp * cs : 0;
+    offS = al(offB + szB);
+    offPart = al(offS + szS);
     offCfl = al(offPart + sizeof(double) * NPART * MON_NRED);
     wsBytes = al(offCfl + sizeof(double) * (size_t)ny * nz);
   }
   ~Plan3D() override {
     if (xcol_sm1) (void)hipFree(xcol_sm1);
     if (xcol_k) (void)hipFree(xcol_k);
+    if (xcol_sig) (void)hipFree(xcol_sig);
     if (unit_x) (void)hipFree(unit_x);
     dx.release();
     dy.release();
@@ -432,6 +462,10 @@ struct Plan3D : PlanBase {
   cx<T>* A(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offA); }
   cx<T>* B(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offB); }
   T* Us(void* ws) const { return reinterpret_cast<T*>((char*)ws + offUs); }
+  // the step's spectral RK4 sum: [5][kept z][kept y][mxp]
+  cx<T>* S(void* ws) const { return reinterpret_cast<cx<T>*>((char*)ws + offS); }
+  long long s_fs() const { return (long long)az.nr * ay.nr * mxp; }
+  long long s_zs() const { return (long long)ay.nr * mxp; }
   double* Part(void* ws) const { return reinterpret_cast<double*>((char*)ws + offPart); }
   // per-row wave-speed maxima of the state the last step wrote (ny*nz)
   double* CflRows(void* ws) const { return reinterpret_cast<double*>((char*)ws + offCfl); }
@@ -625,6 +659,76 @@ struct Plan3D : PlanBase {
     launch_col<CE, CM_FWDINV_CORR>(nz, ay.nr, a, st);
     check_launch("k_col(z corr)");
   }
+  // z pass of an RK4 stage with the spectral sum (CM_FWDINV_DIVACC): as
+  // zpass(true, 5, ...), and S -= w D (stages 1-3) or S = sigma (S - dt/6 D)
+  // transformed back in place of D (stage 4)
+  void zpass_acc(int stage, double dt, const double* dt_dev, void* ws, hipStream_t st,
+                 const int* gate = nullptr) const {
+    ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
+    a.out = reinterpret_cast<cx<CE>*>(B(ws));
+    a.out_fs = a.in_fs;
+    a.out_bs = a.in_bs;
+    a.out_ls = a.in_ls;
+    a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
+    a.nc = 5;
+    a.keep = az.keep();
+    a.tw = dz.tw;
+    a.sig = dz.sig;
+    a.kk = dz.k;
+    a.acc = reinterpret_cast<cx<CE>*>(S(ws));
+    a.acc_fs = s_fs() / PW;
+    a.acc_bs = mxp / PW;
+    a.acc_ls = s_zs() / PW;
+    a.stage = stage;
+    a.dt = dt;
+    a.dt_dev = dt_dev;
+    a.colsig = xcol_sig;
+    a.rowsig = dy.sig;
+    KScope ks("z_acc", st);
+    launch_col<CE, CM_FWDINV_DIVACC>(nz, ay.nr, a, st);
+    check_launch("k_col(z, spectral sum)");
+  }
+  // S = the masked spectrum of U (scaled as the RHS spectra): x and y
+  // forward with the scale alone, then the z forward to S
+  void spec_init(const T* U, void* ws, hipStream_t st, const int* gate = nullptr) const {
+    {
+      XFwdArgs<T> a = xfwd_args(U, 5, 0u, ws, gate);
+      a.sig = dx.one;
+      KScope ks("xfwd_copy", st);
+      launch_xfwd<T, XF_COPY>(nx, ny, nz, a, st);
+      check_launch("k_xfwd(state)");
+    }
+    yfwd(false, 5, 0, ws, st, true, gate);
+    ColArgs<CE> a{};
+    a.active = gate;
+    a.act_all = 1;
+    a.in = reinterpret_cast<cx<CE>*>(B(ws));
+    a.in_fs = (b_fs()) / PW;
+    a.in_bs = b_ys() / PW;
+    a.in_ls = b_zs() / PW;
+    a.out = reinterpret_cast<cx<CE>*>(S(ws));
+    a.out_fs = s_fs() / PW;
+    a.out_bs = mxp / PW;
+    a.out_ls = s_zs() / PW;
+    a.ncol = (mx + PW - 1) / PW;
+    a.ncolw = mxp / PW;
+    a.nc = 5;
+    a.keep = az.keep();
+    a.ikflag = 0;
+    a.tw = dz.tw;
+    a.sig = dz.one;
+    a.kk = dz.k;
+    KScope ks("z_fwd", st);
+    launch_col<CE, CM_FWD_COPY>(nz, ay.nr, a, st);
+    check_launch("k_col(z forward to S)");
+  }
   // y inverse: B (compact) -> A (full)
   void yinv(int nc, void* ws, hipStream_t st, const int* gate = nullptr) const {
     ColArgs<CE> a{};
@@ -716,11 +820,68 @@ struct Plan3D : PlanBase {
   // flux_in: the spectral buffer already holds the x-transformed fluxes of U
   // (left by the previous step's k_xend), so the first x pass is skipped;
   // flux_out: end with k_xend, which leaves those of out for the next step.
+  //
+  // With specacc the RK4 sum is kept in 3D spectral space instead: S starts
+  // as the masked spectrum of U (spec_init; a chained step finds it in S,
+  // left by the previous step's stage 4), each stage's z pass subtracts w D
+  // from it (-D is the spectrum of k), and stage 4's z pass leaves
+  // sigma_x sigma_y sigma_z (S - dt/6 D4) there and transforms it back: the
+  // post-step filter of U + dt/6 (k1 + 2 k2 + 2 k3 + k4), since the k are
+  // band-limited (F k = -D up to rounding).  The x kernels then form the
+  // stage states U + c dt k with no accumulator, and the step has no
+  // separate filter passes.
+  // init_always: a step that computes its own fluxes (and spectrum) does so
+  // even when gate stops it (a run loop's call then leaves them for its
+  // state whether or not a step ran)
   void step(const T* U, T* out, double dt, const double* dt_dev, void* ws, hipStream_t st,
             bool bandlimited = false, const int* gate = nullptr, bool cfl = false, bool flux_in = false,
-            bool flux_out = false) const {
+            bool flux_out = false, bool init_always = false) const {
+    const int* igate = init_always ? nullptr : gate;
+    if (specacc) {
+      if (!flux_in) {
+        spec_init(U, ws, st, igate);
+        xfwd(XF_FLUX3, U, 5, 0u, ws, st, igate);
+      }
+      for (int s = 1; s <= 4; ++s) {
+        yfwd(true, 5, 0, ws, st, false, gate);
+        zpass_acc(s, dt, dt_dev, ws, st, gate);
+        yinv(5, ws, st, gate);
+        if (s == 4) break;
+        XMidArgs<T> m{};
+        m.inv = xinv_args(ws, gate);
+        m.inv.nf = 5;
+        m.inv.out = out;
+        m.inv.un = U;
+        m.inv.stage = s;
+        m.inv.dt = dt;
+        m.inv.dt_dev = dt_dev;
+        m.inv.specacc = 1;
+        m.fwd = xfwd_args(nullptr, 5, 0u, ws, gate);
+        m.fwd_last = m.fwd;
+        KScope ks("xmid_sa", st);
+        launch_xmid<T, 5, XF_FLUX3>(nx, ny, nz, m, st);
+        check_launch("k_xmid");
+      }
+      XInvArgs<T> a = xinv_args(ws, gate);
+      a.nf = 5;
+      a.out = out;
+      a.cfl_rows = cfl ? CflRows(ws) : nullptr;
+      if (flux_out) {
+        XMidArgs<T> m{};
+        m.inv = a;
+        m.fwd = xfwd_args(nullptr, 5, 0u, ws, gate);
+        KScope ks("xend_out", st);
+        launch_xend<T, 5, XI_OUT>(nx, ny, nz, m, st);
+        check_launch("k_xend");
+      } else {
+        KScope ks("xinv_out", st);
+        launch_xinv<T, 5, XI_OUT>(nx, ny, nz, a, st);
+        check_launch("k_xinv");
+      }
+      return;
+    }
     const bool corr = bandlimited && sizeof(T) == 4;
-    if (!flux_in) xfwd(XF_FLUX3, U, 5, 0u, ws, st, gate);
+    if (!flux_in) xfwd(XF_FLUX3, U, 5, 0u, ws, st, igate);
     for (int s = 1; s <= 4; ++s) {
       yfwd(true, 5, 0, ws, st, false, gate);
       zpass(true, 5, 0, ws, st, gate);

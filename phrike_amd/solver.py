@@ -176,7 +176,14 @@ class SpectralSolver3D:
             _lib.call("ph_max_wave_speed3d", plan, _lib.ptr(U), None, _lib.ptr(rs[_lib.PH_RS_DT:]),
                       self.cfl, ws, st)
 
+        calls = [0]
+
         def launch(a, b, n, rs, rf, flags):
+            # every call chains its last step to the next call, which starts
+            # from the buffer holding this call's final state, so the run's
+            # result does not depend on how its steps fall into calls
+            flags |= _lib.PH_STEP_CHAIN | (_lib.PH_STEP_CHAINED if calls[0] else 0)
+            calls[0] += 1
             _lib.call("ph_run3d", plan, _lib.ptr(a), _lib.ptr(b), n, _lib.ptr(rs), _lib.ptr(rf), self.cfl,
                       flags, ws, st)
 

@@ -289,13 +289,15 @@ def test_tgv1024_fp32_matches_512(dev):
     assert d < 1e-5
 
 
-def test_chained_steps_bit_identical(dev):
+def test_chained_steps_agree_with_plain(dev):
     """Steps chained through the C ABI (PH_STEP_CHAIN: the last kernel also
-    forms the next step's fluxes; PH_STEP_CHAINED: the next step uses them)
-    equal plain steps bit for bit, FP64 and FP32, on a 64 x 32 x 16 box; a
-    derivative call between two chained steps makes the second compute its
-    own fluxes (same result); and the solver's own steps take the chained
-    path and agree with the oracle."""
+    forms the next step's fluxes and leaves the filtered state's spectrum;
+    PH_STEP_CHAINED: the next step uses both) agree with plain steps, which
+    transform their input state, to rounding (1e-13 of each field's scale in
+    FP64, 2e-6 in FP32) on a 64 x 32 x 16 box, with a derivative call between
+    two of the steps (the step after it computes its own fluxes and
+    spectrum); and the solver's own steps take the chained path and agree
+    with the oracle."""
     import torch
     from oracle import spectral_np as sn
     from phrike_amd import _lib
@@ -321,7 +323,10 @@ def test_chained_steps_bit_identical(dev):
 
         plain = run(lambda k: BL if k else 0)
         chained = run(lambda k: CH | ((BL | CD) if k else 0))
-        np.testing.assert_array_equal(plain, chained)
+        tol = 1e-13 if prec == "double" else 2e-6
+        for f in range(5):
+            scale = np.abs(plain[f]).max()
+            assert np.abs(plain[f].astype(np.float64) - chained[f]).max() <= tol * scale, (prec, f)
     og = sn.Grid3DOracle(nx, ny, nz, 2 * np.pi, 2 * np.pi, 2 * np.pi,
                          filter_params={"enabled": True, "p": 16, "alpha": 36.0})
     osv = sn.Solver3DOracle(og, sn.Euler3DOracle(1.4), cfl=0.4)
