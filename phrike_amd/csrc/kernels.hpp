@@ -132,7 +132,12 @@ struct XThreads {
 #ifndef PH_XLS
 #define PH_XLS 0
 #endif
-constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : lines; }
+// PH_XODD: an odd stride (lines | 1), so 16-byte accesses of consecutive
+// positions by consecutive lanes fall in distinct banks
+#ifndef PH_XODD
+#define PH_XODD 1
+#endif
+constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : (PH_XODD ? (lines | 1) : lines); }
 // optional position padding of the x kernels' LDS lines (see lds_slot)
 #ifndef PH_XPADB
 #define PH_XPADB 0
@@ -294,6 +299,7 @@ struct ColArgs {
   int ncolw;  // columns a store may cover (the row pitch; tiles then write
               // whole 128-byte lines, zeros past ncol); 0: ncol
   int nc;     // output fields
+  int cpb;    // fields per block along blockIdx.z (0: all in every block; not CM_FWD_DIV3)
   Keep keep;  // retained modes along this axis
   int ikflag; // COPY modes: multiply by i*k
   const cx<T>* tw;
@@ -442,14 +448,18 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   if constexpr (MODE == CM_FWD_DIV3) {
     if (a.symflux && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
   }
+  constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : ((MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) ? 2 : 1);
+  // this block's fields cb..ce-1 (all, or a.cpb of them by blockIdx.z:
+  // launches of few tiles then have more, shorter blocks)
+  const int cb = (MODE != CM_FWD_DIV3 && a.cpb > 0) ? int(blockIdx.z) * a.cpb : 0;
+  const int ce = (MODE != CM_FWD_DIV3 && a.cpb > 0) ? (cb + a.cpb < nc ? cb + a.cpb : nc) : nc;
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
     if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
-    else if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) return (k % 2) * nc + k / 2;
-    else return k;
+    else if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) return (k % 2) * nc + cb + k / 2;
+    else return cb + k;
   };
-  constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : ((MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) ? 2 : 1);
-  const int nsrc = (MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * nc;
+  const int nsrc = (MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * (ce - cb);
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
     const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
@@ -492,7 +502,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   load_pass0<T, N, TM, NT>(tid, loader(0), W);
   lds_barrier();  // twiddles
 #pragma unroll 1
-  for (int c = 0; c < nc; ++c) {
+  for (int c = cb; c < ce; ++c) {
     cx<T>* oc = out + (long long)c * a.out_fs + c0;
     if constexpr (MODE == CM_INV) {
       fwd(INV{});

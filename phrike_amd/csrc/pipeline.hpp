@@ -20,6 +20,11 @@
 #ifndef PH_SPECACC
 #define PH_SPECACC 1
 #endif
+// z passes of fewer tiles than this take one field per block (blockIdx.z),
+// so that few blocks do not leave a nearly empty last round on the CUs
+#ifndef PH_COL_SPLIT_BLOCKS
+#define PH_COL_SPLIT_BLOCKS 4096
+#endif
 
 namespace ph {
 
@@ -179,10 +184,14 @@ void launch_xfwd(int nx, int nrows, int nslab, const XFwdArgs<T>& a, hipStream_t
 }
 
 template <typename T, int MODE>
-void launch_col(int n, int nblocky, const ColArgs<T>& a, hipStream_t st) {
+void launch_col(int n, int nblocky, const ColArgs<T>& a0, hipStream_t st) {
   PH_SWITCH_N(n, ({
     constexpr int TM = ColTile<T, N_, MODE>::TM;
-    dim3 grid((a.ncol + TM - 1) / TM, nblocky);
+    ColArgs<T> a = a0;
+    const int gx = (a.ncol + TM - 1) / TM;
+    // cpb < 0: one field per block when the launch has few tiles
+    if (a.cpb < 0) a.cpb = (long long)gx * nblocky < PH_COL_SPLIT_BLOCKS ? 1 : 0;
+    dim3 grid(gx, nblocky, a.cpb > 0 ? (a.nc + a.cpb - 1) / a.cpb : 1);
     hipLaunchKernelGGL((k_col<T, N_, MODE>), grid, dim3(ColTile<T, N_, MODE>::NT), 0, st, a);
   }));
 }
@@ -575,6 +584,7 @@ struct Plan3D : PlanBase {
     a.tw = dz.tw;
     a.sig = dz.sig;
     a.kk = dz.k;
+    a.cpb = -1;
     KScope ks(div ? "z_div" : "z_copy", st);
     if (div) launch_col<CE, CM_FWDINV_DIV>(nz, ay.nr, a, st);
     else launch_col<CE, CM_FWDINV_COPY>(nz, ay.nr, a, st);
@@ -655,6 +665,7 @@ struct Plan3D : PlanBase {
     a.sm1 = dz.sm1;
     a.colsm1 = xcol_sm1;
     a.rowsm1 = dy.sm1;
+    a.cpb = -1;
     KScope ks("z_corr", st);
     launch_col<CE, CM_FWDINV_CORR>(nz, ay.nr, a, st);
     check_launch("k_col(z corr)");
@@ -691,6 +702,7 @@ struct Plan3D : PlanBase {
     a.dt_dev = dt_dev;
     a.colsig = xcol_sig;
     a.rowsig = dy.sig;
+    a.cpb = -1;
     KScope ks("z_acc", st);
     launch_col<CE, CM_FWDINV_DIVACC>(nz, ay.nr, a, st);
     check_launch("k_col(z, spectral sum)");
