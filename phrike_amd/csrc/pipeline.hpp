@@ -25,6 +25,10 @@
 #ifndef PH_COL_SPLIT_BLOCKS
 #define PH_COL_SPLIT_BLOCKS 4096
 #endif
+// the same for the 3D y inverse pass (1: on)
+#ifndef PH_YINV_SPLIT
+#define PH_YINV_SPLIT 1
+#endif
 
 namespace ph {
 
@@ -762,6 +766,7 @@ struct Plan3D : PlanBase {
     a.tw = dy.tw;
     a.sig = dy.sig;
     a.kk = dy.k;
+    a.cpb = PH_YINV_SPLIT ? -1 : 0;
     KScope ks("yinv", st);
     launch_col<CE, CM_INV>(ny, nz, a, st);
     check_launch("k_col(y inv)");
