@@ -40,9 +40,11 @@ def test_cube_step_fp32(emu):
 
 def test_fp32_filter_correction_form(emu):
     """FP32 step on a band-limited state (a previous step's output) whose kept
-    modes carry a flat random spectrum, so the filter acts on every mode: the
-    correction-form filter equals the direct form up to FP32 rounding; FP64
-    plans ignore the flag; a device dt (dt_dev[0]) equals the host dt."""
+    modes carry a flat random spectrum, so the filter acts on every mode:
+    with and without PH_STEP_BANDLIMITED (which only plans without the
+    spectral RK4 sum use, for the correction-form filter) within FP32
+    rounding of FP64; FP64 plans ignore the flag; a device dt (dt_dev[0])
+    equals the host dt."""
     from emu_lib import Emu3D
     d = golden("tgv3d_16.npz")
     dt = float(d["dt"])
