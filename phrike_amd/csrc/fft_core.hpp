@@ -247,7 +247,14 @@ struct LdsSize {
 // entries serves a length-N transform with TWS = 2: the single-row x kernels
 // share one table between their N/2-point transforms and the real-to-complex
 // split)
-template <typename T, int N, int L, int DIR, int R, int NS, int LS = L, int PADB = 0, int TWS = 1>
+// Slot of twiddle m in a table stored with TSW = 1: entries swizzled within
+// aligned blocks of 16 (m ^ (m / 16 mod 16)), so reads at strides that are
+// multiples of 16 entries (the early passes of a transform) fall in distinct
+// LDS banks
+template <int TSW>
+__device__ __forceinline__ int twslot(int m) { return TSW ? (m ^ ((m >> 4) & 15)) : m; }
+
+template <typename T, int N, int L, int DIR, int R, int NS, int LS = L, int PADB = 0, int TWS = 1, int TSW = 0>
 __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int b, cx<T>* v) {
   const int l = b % L, j = b / L;
 #pragma unroll
@@ -256,7 +263,7 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
     const int k = j % NS;
 #pragma unroll
     for (int r = 1; r < R; ++r) {
-      cx<T> w = tw[TWS * (r * k * (N / (NS * R)))];
+      cx<T> w = tw[twslot<TSW>(TWS * (r * k * (N / (NS * R))))];
       if constexpr (DIR > 0) w.y = -w.y;
       v[r] = cmul(v[r], w);
     }
@@ -267,7 +274,8 @@ __device__ __forceinline__ void bfly_load(const cx<T>* buf, const cx<T>* tw, int
 // one Stockham pass, LDS -> LDS (in place; ends with a barrier).  Lines are
 // interleaved with stride LS >= L (padding keeps 16-byte accesses free of
 // bank conflicts when L is not a multiple of 8).
-template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1>
+template <typename T, int N, int L, int DIR, int NT, int P, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1,
+          int TSW = 0>
 __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
   constexpr int R = PL::radix(P);
   constexpr int NS = PassNs<N, P, PL>::value;
@@ -277,7 +285,7 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
-    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS, PADB, TWS>(buf, tw, b, v[q]);
+    if (NB % NT == 0 || b < NB) bfly_load<T, N, L, DIR, R, NS, LS, PADB, TWS, TSW>(buf, tw, b, v[q]);
   }
   lds_barrier();
 #pragma unroll
@@ -295,20 +303,21 @@ __device__ __forceinline__ void pass_lds(cx<T>* buf, const cx<T>* tw, int tid) {
 }
 
 template <typename T, int N, int L, int DIR, int NT, int P, int PEND, int LS = L, int PADB = 0,
-          class PL = Plan<N>, int TWS = 1>
+          class PL = Plan<N>, int TWS = 1, int TSW = 0>
 __device__ __forceinline__ void passes_lds_range(cx<T>* buf, const cx<T>* tw, int tid) {
   if constexpr (P < PEND) {
-    pass_lds<T, N, L, DIR, NT, P, LS, PADB, PL, TWS>(buf, tw, tid);
-    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB, PL, TWS>(buf, tw, tid);
+    pass_lds<T, N, L, DIR, NT, P, LS, PADB, PL, TWS, TSW>(buf, tw, tid);
+    passes_lds_range<T, N, L, DIR, NT, P + 1, PEND, LS, PADB, PL, TWS, TSW>(buf, tw, tid);
   }
 }
 
 // Full transform of the L lines in buf (line stride LS), result in buf
 // (natural order).  Caller must have synchronised after filling buf; ends
 // with a barrier.
-template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1>
+template <typename T, int N, int L, int DIR, int NT, int LS = L, int PADB = 0, class PL = Plan<N>, int TWS = 1,
+          int TSW = 0>
 __device__ __forceinline__ void fft_lds(cx<T>* buf, const cx<T>* tw, int tid) {
-  passes_lds_range<T, N, L, DIR, NT, 0, PL::npass, LS, PADB, PL, TWS>(buf, tw, tid);
+  passes_lds_range<T, N, L, DIR, NT, 0, PL::npass, LS, PADB, PL, TWS, TSW>(buf, tw, tid);
 }
 
 // Full transform with the last pass left in registers v[q][r]; output r of

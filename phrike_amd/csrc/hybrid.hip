@@ -160,7 +160,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
     }
   }
   lds_barrier();
-  fft_lds<T, NX, 4, -1, NT, xstride(4), PH_XPADB, XPlan<NX>>(buf, tw, tid);
+  fft_lds<T, NX, 4, -1, NT, xstride(4), PH_XPADB, XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
   // i k (or 1) * mask * sigma / nx; both rows of the pair share the multiplier,
   // which keeps the spectrum Hermitian per row (k(-m) = -k(m), k = 0 at Nyquist)
   for (int i = tid; i < 4 * NX; i += NT) {
@@ -171,7 +171,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
     buf[xslot<4>(n, c)] = z;
   }
   lds_barrier();
-  fft_lds<T, NX, 4, +1, NT, xstride(4), PH_XPADB, XPlan<NX>>(buf, tw, tid);
+  fft_lds<T, NX, 4, +1, NT, xstride(4), PH_XPADB, XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
 #pragma unroll
   for (int pi = 0; pi < PPT; ++pi) {
     const int p = tid + pi * NT;

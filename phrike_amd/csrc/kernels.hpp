@@ -152,13 +152,18 @@ constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value;
 #ifndef PH_XTW_GLOBAL
 #define PH_XTW_GLOBAL 0
 #endif
+// The LDS copy is stored swizzled (twslot<1>; PH_XTW_SWZ 0: plain).
 #if PH_XTW_GLOBAL
+#define PH_XTW_SWZ 0
 #define PH_XTW_DECL(T, NX) const cx<T>* tw = nullptr
 #define PH_XTW_FILL(src, NX, NT, tid) tw = (src)
 #else
+#ifndef PH_XTW_SWZ
+#define PH_XTW_SWZ 1
+#endif
 #define PH_XTW_DECL(T, NX) __shared__ cx<T> tw[NX]
 #define PH_XTW_FILL(src, NX, NT, tid) \
-  for (int i_ = (tid); i_ < (NX); i_ += (NT)) tw[i_] = (src)[i_]
+  for (int i_ = (tid); i_ < (NX); i_ += (NT)) tw[twslot<PH_XTW_SWZ>(i_)] = (src)[i_]
 #endif
 // k_xmid loads the RK combine's inputs before its inverse transform (1: U and
 // the accumulator, 2: U only, 0: none) so their latency overlaps it
@@ -1053,11 +1058,12 @@ __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
       b.y = T(0);
     }
     const cx<T> bc = mk(b.x, -b.y);
-    const cx<T> d1 = cmul(mk(tw[k].x, -tw[k].y), a - bc);  // W^-k (X[k] - X*[M-k])
+    const cx<T> w1 = tw[twslot<PH_XTW_SWZ>(k)], w2 = tw[twslot<PH_XTW_SWZ>(k2)];
+    const cx<T> d1 = cmul(mk(w1.x, -w1.y), a - bc);  // W^-k (X[k] - X*[M-k])
     buf[xslot<LINES>(k, l)] = (a + bc) + mk(-d1.y, d1.x);
     if (k != 0 && k2 != k) {
       const cx<T> ac = mk(a.x, -a.y);
-      const cx<T> d2 = cmul(mk(tw[k2].x, -tw[k2].y), b - ac);
+      const cx<T> d2 = cmul(mk(w2.x, -w2.y), b - ac);
       buf[xslot<LINES>(k2, l)] = (b + ac) + mk(-d2.y, d2.x);
     }
   }
@@ -1089,7 +1095,7 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
           const cx<T> z2 = buf[xslot<LX>((M - k) & (M - 1), l)];
           const cx<T> e = mk(T(0.5) * (a.x + z2.x), T(0.5) * (a.y - z2.y));
           const cx<T> h = mk(T(0.5) * (a.x - z2.x), T(0.5) * (a.y + z2.y));  // (Z[k] - Z*[M-k]) / 2
-          cx<T> X = e + cmul(tw[k], mk(h.y, -h.x));                           // E + W^k (-i h)
+          cx<T> X = e + cmul(tw[twslot<PH_XTW_SWZ>(k)], mk(h.y, -h.x));       // E + W^k (-i h)
           X = cscale(X, sig[k]);
           if (IK && ((ikmask >> ((l / GL) * GL)) & 1u)) X = mul_ik(X, kk[k]);  // i k of line l's group
           ch_set(c, i, X);
@@ -1213,7 +1219,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
         }
       }
     lds_barrier();
-    fft_lds<T, M, LG, -1, NT, xstride(LG), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
+    fft_lds<T, M, LG, -1, NT, xstride(LG), PH_XPADB, XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
     xr_split_store<T, NX, DA, LG, LX, MODE != XF_FLUX3>(buf, tw, dst, a.out_fs, LG, sig, kk, a.ikmask, tid);
     lds_barrier();
     return;
@@ -1263,7 +1269,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
       }
     }
     lds_barrier();
-    fft_lds<T, M, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<M>, 2>(buf, tw, tid);
+    fft_lds<T, M, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
     xr_split_store<T, NX, DA, LX, LX, IK>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
                                           (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();
@@ -1351,7 +1357,7 @@ __device__ __forceinline__ void xr_inverse(const XRSpec<T, NX, DA, NC>& sp, cx<T
   lds_barrier();  // also covers the block's twiddle fill
   xr_pack<T, NX, NC, G::NT>(buf, tw, tid);
   lds_barrier();
-  fft_lds<T, G::M, NC, +1, G::NT, xstride(NC), PH_XPADB, XPlan<G::M>, 2>(buf, tw, tid);
+  fft_lds<T, G::M, NC, +1, G::NT, xstride(NC), PH_XPADB, XPlan<G::M>, 2, PH_XTW_SWZ>(buf, tw, tid);
 }
 
 template <typename T, int NX, bool DA, int NC, int MODE>
