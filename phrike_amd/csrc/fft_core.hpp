@@ -234,9 +234,15 @@ struct LastRegs {
 // LDS slot of (position, line) for line stride LS; PADB > 0 inserts one
 // empty position after every 2^PADB positions (breaks power-of-two strides
 // across LDS banks).
+// PADB < 0 (LS a power of two): no padding; line l of position pos is
+// rotated by pos LS / W, W = 16 (PADB = -1, 16-byte elements) or 32 (-2,
+// 8-byte elements) slots of a 256-byte bank window, so that W consecutive
+// positions of one line, and W consecutive slots of whole positions, each
+// fall in distinct banks
 template <int LS, int PADB>
 __device__ __forceinline__ int lds_slot(int pos, int l) {
-  return (PADB > 0 ? pos + (pos >> PADB) : pos) * LS + l;
+  if constexpr (PADB < 0) return pos * LS + ((l + ((pos * LS) >> (PADB == -1 ? 4 : 5))) & (LS - 1));
+  else return (PADB > 0 ? pos + (pos >> PADB) : pos) * LS + l;
 }
 template <int N, int LS, int PADB>
 struct LdsSize {

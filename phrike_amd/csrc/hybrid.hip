@@ -104,7 +104,7 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
   constexpr int NT = XThreads<T, NX>::value;
   constexpr int PPT = (2 * NX + NT - 1) / NT;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xbuf_size<NX, 4>()];
+  __shared__ cx<T> buf[xbuf_size<NX, 4, T>()];
   T* bufr = reinterpret_cast<T*>(buf);
   const int tid = threadIdx.x;
   const int run = blockIdx.y;
@@ -151,34 +151,34 @@ __global__ __launch_bounds__((XThreads<T, NX>::value)) void k_hx(HxArgs<T> a) {
       const T fy[4] = {my, my * ux, my * uy + pr, ep * uy};
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        bufr[2 * xslot<4>(x, c) + line] = fx[c];
+        bufr[2 * xslot<4, T>(x, c) + line] = fx[c];
         a.fy[c * a.fs + base + p] = fy[c];
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) bufr[2 * xslot<4>(x, c) + line] = q[c];
+      for (int c = 0; c < 4; ++c) bufr[2 * xslot<4, T>(x, c) + line] = q[c];
     }
   }
   lds_barrier();
-  fft_lds<T, NX, 4, -1, NT, xstride(4), PH_XPADB, XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
+  fft_lds<T, NX, 4, -1, NT, xstride(4, sizeof(cx<T>)), xpadb(4, sizeof(cx<T>)), XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
   // i k (or 1) * mask * sigma / nx; both rows of the pair share the multiplier,
   // which keeps the spectrum Hermitian per row (k(-m) = -k(m), k = 0 at Nyquist)
   for (int i = tid; i < 4 * NX; i += NT) {
     const int n = i >> 2, c = i & 3;
-    cx<T> z = buf[xslot<4>(n, c)];
+    cx<T> z = buf[xslot<4, T>(n, c)];
     z = cscale(z, a.fac[n]);
     if constexpr (MODE == HX_RHS) z = mul_ik(z, a.kx[n]);
-    buf[xslot<4>(n, c)] = z;
+    buf[xslot<4, T>(n, c)] = z;
   }
   lds_barrier();
-  fft_lds<T, NX, 4, +1, NT, xstride(4), PH_XPADB, XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
+  fft_lds<T, NX, 4, +1, NT, xstride(4, sizeof(cx<T>)), xpadb(4, sizeof(cx<T>)), XPlan<NX>, 1, PH_XTW_SWZ>(buf, tw, tid);
 #pragma unroll
   for (int pi = 0; pi < PPT; ++pi) {
     const int p = tid + pi * NT;
     if (p >= 2 * NX) continue;
     const int line = p / NX, x = p - line * NX;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) a.dfx[c * a.fs + base + p] = bufr[2 * xslot<4>(x, c) + line];
+    for (int c = 0; c < 4; ++c) a.dfx[c * a.fs + base + p] = bufr[2 * xslot<4, T>(x, c) + line];
   }
 }
 

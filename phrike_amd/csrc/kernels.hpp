@@ -137,15 +137,30 @@ struct XThreads {
 #ifndef PH_XODD
 #define PH_XODD 1
 #endif
-constexpr int xstride(int lines) { return PH_XLS > 0 ? PH_XLS : (PH_XODD ? (lines | 1) : lines); }
+// PH_XROT: power-of-two line counts keep their stride and rotate the lines
+// of each position instead (lds_slot with PADB < 0), for 16-byte elements
+// (bit 0: FP64) and 8-byte elements (bit 1: FP32)
+#ifndef PH_XROT
+#define PH_XROT 1
+#endif
 // optional position padding of the x kernels' LDS lines (see lds_slot)
 #ifndef PH_XPADB
 #define PH_XPADB 0
 #endif
-template <int LINES>
-__device__ __forceinline__ int xslot(int pos, int l) { return lds_slot<xstride(LINES), PH_XPADB>(pos, l); }
-template <int NX, int LINES>
-constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES), PH_XPADB>::value; }
+constexpr bool xrot(int lines, int es) {
+  return PH_XLS == 0 && lines > 1 && (lines & (lines - 1)) == 0 &&
+         ((es >= 16 && (PH_XROT & 1)) || (es < 16 && (PH_XROT & 2)));
+}
+constexpr int xstride(int lines, int es) {
+  return PH_XLS > 0 ? PH_XLS : (xrot(lines, es) ? lines : (PH_XODD ? (lines | 1) : lines));
+}
+constexpr int xpadb(int lines, int es) { return xrot(lines, es) ? (es >= 16 ? -1 : -2) : PH_XPADB; }
+template <int LINES, typename T>
+__device__ __forceinline__ int xslot(int pos, int l) {
+  return lds_slot<xstride(LINES, sizeof(cx<T>)), xpadb(LINES, sizeof(cx<T>))>(pos, l);
+}
+template <int NX, int LINES, typename T>
+constexpr int xbuf_size() { return LdsSize<NX, xstride(LINES, sizeof(cx<T>)), xpadb(LINES, sizeof(cx<T>))>::value; }
 
 // Twiddles of the x kernels: an LDS copy per block (PH_XTW_GLOBAL 0), or read
 // where they lie in global memory through the L1 cache (1).
@@ -978,8 +993,10 @@ __device__ __forceinline__ void xr_store_pairs(T* row, int tid, const cx<T> (&v)
   }
 }
 // LDS of a packed row: M transform points and the Nyquist slot M
-template <int NX, int LINES>
-constexpr int xrbuf_size() { return LdsSize<NX / 2 + 1, xstride(LINES), PH_XPADB>::value; }
+template <int NX, int LINES, typename T>
+constexpr int xrbuf_size() {
+  return LdsSize<NX / 2 + 1, xstride(LINES, sizeof(cx<T>)), xpadb(LINES, sizeof(cx<T>))>::value;
+}
 
 template <typename T, int NX, bool DA>
 __device__ __forceinline__ void xr_tables(const T* sig, const T* kk, T* s_sig, T* s_k, int tid) {
@@ -1024,7 +1041,7 @@ struct XRSpec {
 #pragma unroll
         for (int i = 0; i < G::K; ++i) {
           const int k = m * G::K + i;
-          if (k < G::MX) buf[xslot<NCI>(k, l)] = ch_get(a[q], i);
+          if (k < G::MX) buf[xslot<NCI, T>(k, l)] = ch_get(a[q], i);
         }
       }
     }
@@ -1035,7 +1052,7 @@ struct XRSpec {
         const int t = tid + i * G::NT;
         if ((NCI * ZB) % G::NT != 0 && t >= NCI * ZB) break;
         const int l = t / ZB, k = G::MX + (t - l * ZB);
-        buf[xslot<NCI>(k, l)] = mk(T(0), T(0));
+        buf[xslot<NCI, T>(k, l)] = mk(T(0), T(0));
       }
     }
   }
@@ -1052,7 +1069,7 @@ __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
     const int t = tid + i * NT;
     if ((LINES * H) % NT != 0 && t >= LINES * H) break;
     const int l = t / H, k = t - l * H, k2 = M - k;
-    cx<T> a = buf[xslot<LINES>(k, l)], b = buf[xslot<LINES>(k2, l)];
+    cx<T> a = buf[xslot<LINES, T>(k, l)], b = buf[xslot<LINES, T>(k2, l)];
     if (k == 0) {  // X[0], X[M]: real parts
       a.y = T(0);
       b.y = T(0);
@@ -1060,11 +1077,11 @@ __device__ __forceinline__ void xr_pack(cx<T>* buf, const cx<T>* tw, int tid) {
     const cx<T> bc = mk(b.x, -b.y);
     const cx<T> w1 = tw[twslot<PH_XTW_SWZ>(k)], w2 = tw[twslot<PH_XTW_SWZ>(k2)];
     const cx<T> d1 = cmul(mk(w1.x, -w1.y), a - bc);  // W^-k (X[k] - X*[M-k])
-    buf[xslot<LINES>(k, l)] = (a + bc) + mk(-d1.y, d1.x);
+    buf[xslot<LINES, T>(k, l)] = (a + bc) + mk(-d1.y, d1.x);
     if (k != 0 && k2 != k) {
       const cx<T> ac = mk(a.x, -a.y);
       const cx<T> d2 = cmul(mk(w2.x, -w2.y), b - ac);
-      buf[xslot<LINES>(k2, l)] = (b + ac) + mk(-d2.y, d2.x);
+      buf[xslot<LINES, T>(k2, l)] = (b + ac) + mk(-d2.y, d2.x);
     }
   }
 }
@@ -1091,8 +1108,8 @@ __device__ __forceinline__ void xr_split_store(const cx<T>* buf, const cx<T>* tw
       for (int i = 0; i < G::K; ++i) {
         const int k = m * G::K + i;
         if (k < G::MX) {
-          const cx<T> a = buf[xslot<LX>(k & (M - 1), l)];
-          const cx<T> z2 = buf[xslot<LX>((M - k) & (M - 1), l)];
+          const cx<T> a = buf[xslot<LX, T>(k & (M - 1), l)];
+          const cx<T> z2 = buf[xslot<LX, T>((M - k) & (M - 1), l)];
           const cx<T> e = mk(T(0.5) * (a.x + z2.x), T(0.5) * (a.y - z2.y));
           const cx<T> h = mk(T(0.5) * (a.x - z2.x), T(0.5) * (a.y + z2.y));  // (Z[k] - Z*[M-k]) / 2
           cx<T> X = e + cmul(tw[twslot<PH_XTW_SWZ>(k)], mk(h.y, -h.x));       // E + W^k (-i h)
@@ -1213,13 +1230,13 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
         if (G::PALL || p < M) {
 #pragma unroll
           for (int l = 0; l < LX; ++l)
-            buf[xslot<LG>(p, g * LX + l)] =
+            buf[xslot<LG, T>(p, g * LX + l)] =
                 MODE == XF_FLUX3 ? mk(euler3_flux_sym<T>(pr[j][0], g * LX + l), euler3_flux_sym<T>(pr[j][1], g * LX + l))
                                  : mk(euler_flux<T, NC>(pr[j][0], g, l), euler_flux<T, NC>(pr[j][1], g, l));
         }
       }
     lds_barrier();
-    fft_lds<T, M, LG, -1, NT, xstride(LG), PH_XPADB, XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
+    fft_lds<T, M, LG, -1, NT, xstride(LG, sizeof(cx<T>)), xpadb(LG, sizeof(cx<T>)), XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
     xr_split_store<T, NX, DA, LG, LX, MODE != XF_FLUX3>(buf, tw, dst, a.out_fs, LG, sig, kk, a.ikmask, tid);
     lds_barrier();
     return;
@@ -1264,12 +1281,12 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
               v[r] = euler_flux<T, NC>(pr[j][r], g, l);
             }
           }
-          buf[xslot<LX>(p, l)] = mk(v[0], v[1]);
+          buf[xslot<LX, T>(p, l)] = mk(v[0], v[1]);
         }
       }
     }
     lds_barrier();
-    fft_lds<T, M, LX, -1, NT, xstride(LX), PH_XPADB, XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
+    fft_lds<T, M, LX, -1, NT, xstride(LX, sizeof(cx<T>)), xpadb(LX, sizeof(cx<T>)), XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
     xr_split_store<T, NX, DA, LX, LX, IK>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
                                           (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();
@@ -1282,7 +1299,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   using S = XFwdShape<MODE>;
   constexpr int NT = G::NT, NC = S::NC, PJ = G::PJ;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xrbuf_size<NX, XMerge<T, NX, MODE>::lines>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, XMerge<T, NX, MODE>::lines, T>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
@@ -1357,7 +1374,7 @@ __device__ __forceinline__ void xr_inverse(const XRSpec<T, NX, DA, NC>& sp, cx<T
   lds_barrier();  // also covers the block's twiddle fill
   xr_pack<T, NX, NC, G::NT>(buf, tw, tid);
   lds_barrier();
-  fft_lds<T, G::M, NC, +1, G::NT, xstride(NC), PH_XPADB, XPlan<G::M>, 2, PH_XTW_SWZ>(buf, tw, tid);
+  fft_lds<T, G::M, NC, +1, G::NT, xstride(NC, sizeof(cx<T>)), xpadb(NC, sizeof(cx<T>)), XPlan<G::M>, 2, PH_XTW_SWZ>(buf, tw, tid);
 }
 
 template <typename T, int NX, bool DA, int NC, int MODE>
@@ -1365,7 +1382,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   using G = XRGeo<T, NX, DA>;
   constexpr int NT = G::NT, PJ = G::PJ;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xrbuf_size<NX, NC>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, NC, T>()];
   const int tid = threadIdx.x;
   const int slab = blockIdx.y;
   if (gated_off(a.active, a.act_all, slab)) return;
@@ -1391,7 +1408,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       const int p = G::point(tid, j);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC, T>(p, c)] : mk(T(0), T(0));
         v[j][0][c] = z.x;
         v[j][1][c] = z.y;
       }
@@ -1561,7 +1578,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   PH_XTW_DECL(T, NX);
   constexpr int LB0 = NC > XMerge<T, NX, FMODE>::lines ? NC : XMerge<T, NX, FMODE>::lines;
   constexpr int LB = LB0 > XFwdShape<LMODE>::LX ? LB0 : XFwdShape<LMODE>::LX;
-  __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, LB, T>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
@@ -1627,7 +1644,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       T v[2][NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC, T>(p, c)] : mk(T(0), T(0));
         v[0][c] = z.x;
         v[1][c] = z.y;
       }
@@ -1684,7 +1701,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
   static_assert(MODE == XI_OUT || MODE == XI_ADD, "the filter's last pass writes or adds");
   constexpr int LB = NC > XMerge<T, NX, FMODE>::lines ? NC : XMerge<T, NX, FMODE>::lines;
   PH_XTW_DECL(T, NX);
-  __shared__ cx<T> buf[xrbuf_size<NX, LB>()];
+  __shared__ cx<T> buf[xrbuf_size<NX, LB, T>()];
   __shared__ T s_sig[G::MX], s_k[G::MX];
   const XInvArgs<T>& a = m.inv;
   const int tid = threadIdx.x;
@@ -1714,7 +1731,7 @@ __global__ __launch_bounds__((XRThreads<T, NX>::value), PH_X_WAVES_PER_EU) void 
       const int p = G::point(tid, j);
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC>(p, c)] : mk(T(0), T(0));
+        const cx<T> z = (G::PALL || p < G::M) ? buf[xslot<NC, T>(p, c)] : mk(T(0), T(0));
         o[c][j] = MODE == XI_ADD ? mk(o[c][j].x + z.x, o[c][j].y + z.y) : z;
         q[j][0][c] = o[c][j].x;
         q[j][1][c] = o[c][j].y;
