@@ -679,6 +679,8 @@ struct Plan3D : PlanBase {
   // transformed back in place of D (stage 4)
   void zpass_acc(int stage, double dt, const double* dt_dev, void* ws, hipStream_t st,
                  const int* gate = nullptr) const {
+    // the kernel's sum loop covers the 2N/3 + 1 kept z modes of the 2/3 rule
+    if (!specacc || az.nr > 2 * (nz / 3) + 1) throw Error(-9, "spectral RK4 sum needs the 2/3 rule");
     ColArgs<CE> a{};
     a.active = gate;
     a.act_all = 1;
