@@ -407,6 +407,10 @@ struct ColTile {
   static constexpr int NT = clampi((N * TM) / PERT, 64, 1024);
 };
 
+// loads of the spectral RK4 sum issued together per thread (CM_FWDINV_DIVACC)
+#ifndef PH_ZACC_BATCH
+#define PH_ZACC_BATCH 4
+#endif
 #ifndef PH_COL_WAVES_PER_EU
 #define PH_COL_WAVES_PER_EU 2
 #endif
@@ -511,9 +515,11 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
     });
   };
   int k = 0;
-  auto fwd = [&](auto dir) {  // next global-source transform into X
+  // next global-source transform into X; with pf its last passes overlap the
+  // loads of the following transform's first pass (into W)
+  auto fwd = [&](auto dir, bool pf = true) {
     constexpr int D = decltype(dir)::value;
-    const bool more = k + 1 < nsrc;
+    const bool more = pf && k + 1 < nsrc;
     fft_regs_pipe<T, N, TM, D, NT>(buf, tw, tid, X, W, more, loader(more ? k + 1 : k));
     ++k;
   };
@@ -648,7 +654,9 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       fwd(FWD{});
       if constexpr (DIV) {
         keep_first();
-        fwd(FWD{});
+        // CM_FWDINV_DIVACC loads the next field's first pass after the RK4
+        // sum instead, so W is free during it
+        fwd(FWD{}, MODE != CM_FWDINV_DIVACC);
       }
       each([&](int q, int r, int l, int n, bool ok) {
         cx<T> v = mk(T(0), T(0));
@@ -679,14 +687,32 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         const T sy = a.rowsig[blockIdx.y];
         cx<T>* sc = a.acc + (long long)c * a.acc_fs + (long long)blockIdx.y * a.acc_bs + c0;
         constexpr int NTASK = ((N / 3) * 2 + 1) * TM;  // kept positions at most (2/3 rule)
+        constexpr int NIT = (NTASK + NT - 1) / NT;
+        // the thread's loads of S in batches of SB issued together (a store
+        // between two loads would order them: the compiler cannot tell the
+        // slots apart), SB bounded so the batch fits next to the prefetch
+        // registers of the next transform
+        constexpr int SB = PH_ZACC_BATCH < NIT ? PH_ZACC_BATCH : NIT;
 #pragma unroll
-        for (int i = 0; i < (NTASK + NT - 1) / NT; ++i) {
+        for (int i0 = 0; i0 < NIT; i0 += SB) {
+          cx<T> s0[SB];
+#pragma unroll
+          for (int ii = 0; ii < SB; ++ii) {
+            const int t = tid + (i0 + ii) * NT;
+            const int j = t / TM, l = t - j * TM;
+            s0[ii] = (i0 + ii < NIT && j < kp.nr && c0 + l < ncolw) ? ld(sc + (long long)j * a.acc_ls + l)
+                                                                    : mk(T(0), T(0));
+          }
+#pragma unroll
+          for (int ii = 0; ii < SB; ++ii) {
+          const int i = i0 + ii;
+          if (i >= NIT) break;
           const int t = tid + i * NT;
           const int j = t / TM, l = t - j * TM;
           if (j < kp.nr && c0 + l < ncolw) {
             const int n = kp.natural(j, N);
             cx<T>* sp = sc + (long long)j * a.acc_ls + l;
-            const cx<T> s1 = ld(sp) - cscale(buf[n * TM + l], w);
+            const cx<T> s1 = s0[ii] - cscale(buf[n * TM + l], w);
             if (last) {
               const cx<T> u = cscale(s1, (a.colsig[c0 + l] * sy) * a.sig[j]);
               st(sp, u);
@@ -695,8 +721,10 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
               st(sp, s1);
             }
           }
+          }
         }
         if (last) lds_barrier();
+        if (k < nsrc) load_pass0<T, N, TM, NT>(tid, loader(k), W);
       }
       fft_regs<T, N, TM, +1, NT>(buf, tw, tid, X);
       each([&](int q, int r, int l, int n, bool ok) {
