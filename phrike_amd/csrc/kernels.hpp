@@ -2,19 +2,25 @@
 //
 // One RHS evaluation in 3D is five launches (the 2D form skips the z pass):
 //
-//   k_xfwd   flux (5 conserved -> 15 fluxes, in registers) + real-to-complex
-//            FFT along x of one y-row (a half-length complex FFT), keeps
-//            |mx| <= Nx/3, scales by sigma_x/(Nx*Ny*Nz) and multiplies the
-//            x-fluxes by i*kx                         -> A[15][z][y][mx]
+//   k_xfwd   fluxes (in registers) + real-to-complex FFT along x of one
+//            y-row (a half-length complex FFT), keeps |mx| <= Nx/3, scales by
+//            sigma_x/(Nx*Ny*Nz).  3D Euler: the 12 distinct fluxes (the
+//            momentum flux is symmetric), without i kx -> A[12][z][y][mx];
+//            2D Euler and MHD: every flux, x-fluxes times i kx
 //   k_col    y pass: forward FFT along y of column tiles, keeps |my| <= Ny/3,
-//            forms G = sigma_y (Fx^ + i ky Fy^) and sigma_y Fz^
-//                                                     -> B[10][z][my][mx]
+//            forms G = sigma_y (i kx Fx^ + i ky Fy^) (2D / MHD: Fx^ already
+//            carries i kx) and sigma_y Fz^          -> B[10][z][my][mx]
 //   k_col    z pass: forward FFT along z of G and Fz^, D = sigma_z (G + i kz
-//            Fz^) on |mz| <= Nz/3, zero elsewhere, inverse FFT along z
-//                                                     -> C[5][z][my][mx] (in B)
-//   k_col    y inverse: zero-filled inverse FFT along y -> D[5][z][y][mx] (in A)
-//   k_xinv   complex-to-real inverse along x of one row, then the RK4
-//            stage combine (or plain output) in registers.
+//            Fz^) on |mz| <= Nz/3, zero elsewhere (3D Euler steps: also the
+//            RK4 sum in spectral space, CM_FWDINV_DIVACC), inverse FFT along
+//            z                                      -> [5][z][my][mx] (in B)
+//   k_col    y inverse: zero-filled inverse FFT along y -> [5][z][y][mx] (in A)
+//   k_xinv   complex-to-real inverse along x of one row, then the output (or
+//            a stage combine) in registers.
+//
+// In a step, k_xmid joins the x inverse of one stage to the next stage's
+// fluxes and x forward (the stage states never reach HBM), and k_xend writes
+// the new state and forms the next step's fluxes and x forward.
 //
 // Because differentiation, the 2/3 mask and the filter are all diagonal in
 // Fourier space, dx Fx + dy Fy + dz Fz is formed once in spectral space and
@@ -22,7 +28,8 @@
 // masked derivatives, with 2/3 of the modes never written to HBM.
 //
 // All spectral buffers hold complex cx<T>; row pitch mxp >= mx keeps rows
-// aligned.  Every kernel uses 256 threads (four wave64s).
+// aligned.  The x kernels take NX/2 (FP64) or NX/4 (FP32) threads per row
+// (at least 64); the column kernels N * TM / 8.
 #pragma once
 #include <type_traits>
 

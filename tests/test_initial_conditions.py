@@ -1,5 +1,5 @@
-"""Properties of the decaying-turbulence initial state (
-random-phase solenoidal velocity, E(k) ~ k^4 exp(-2 (k/4)^2), rms Mach 0.3,
+"""Properties of the decaying-turbulence initial state (the FP32-vs-FP64
+turbulence workload: random-phase solenoidal velocity, E(k) ~ k^4 exp(-2 (k/4)^2), rms Mach 0.3,
 rho = 1 + 1% Gaussian noise, uniform p) checked from the mathematics: the
 spectral divergence vanishes, the rms Mach number is 0.3, the shell spectrum
 peaks near k = 4 and follows the k^4 exp(-2 (k/4)^2) shape, and the seeds
