@@ -90,6 +90,34 @@ __device__ __forceinline__ int sym_g(int k) { return int((kEulerSymG >> (4 * k))
 __device__ __forceinline__ bool sym_keep(int k) { return (kEulerSymKeep >> k) & 1u; }
 __device__ __forceinline__ bool sym_first_x(int k) { return (kEulerSymFirstX >> k) & 1u; }
 
+// The y pass's order of the 15 distinct ideal-MHD fluxes (mhd3_flux_sym's u)
+// and, per step: the component c of a sigma F(Fz[c]) output (field 8 + c; 15
+// none) and its sign; up to two G outputs (component, 15 none) with a form
+// (1: i kx kept + i ky X, 2: i kx X + i ky kept, 3: i kx X, 4: i ky X) and a
+// sign; whether the step's spectrum is kept.  Fz[Bz] = 0 is written as zeros.
+constexpr unsigned long long kMhdSymSeq =
+    (0ull << 0) | (1ull << 4) | (2ull << 8) | (6ull << 12) | (3ull << 16) | (4ull << 20) | (7ull << 24) |
+    (8ull << 28) | (5ull << 32) | (9ull << 36) | (10ull << 40) | (11ull << 44) | (12ull << 48) | (13ull << 52) |
+    (14ull << 56);
+constexpr unsigned long long kMhdSymZ =
+    (15ull << 0) | (15ull << 4) | (0ull << 8) | (15ull << 12) | (15ull << 16) | (15ull << 20) | (1ull << 24) |
+    (2ull << 28) | (3ull << 32) | (15ull << 36) | (15ull << 40) | (4ull << 44) | (15ull << 48) | (5ull << 52) |
+    (6ull << 56);
+constexpr unsigned kMhdSymZNeg = (1u << 13) | (1u << 14);
+constexpr unsigned long long kMhdSymG =
+    (15ull << 0) | (0ull << 4) | (15ull << 8) | (15ull << 12) | (1ull << 16) | (2ull << 20) | (15ull << 24) |
+    (3ull << 28) | (15ull << 32) | (15ull << 36) | (4ull << 40) | (15ull << 44) | (5ull << 48) | (15ull << 52) |
+    (7ull << 56);
+constexpr unsigned long long kMhdSymGForm =
+    (1ull << 4) | (2ull << 16) | (1ull << 20) | (1ull << 28) | (1ull << 40) | (4ull << 48) | (1ull << 56);
+constexpr unsigned kMhdSymGNeg = 1u << 12;  // G[Bx] = i ky (-(ux By - uy Bx))
+constexpr int kMhdSymG2Step = 12, kMhdSymG2 = 6, kMhdSymG2Form = 3;  // G[By] = i kx (ux By - uy Bx)
+constexpr unsigned kMhdSymKeep = (1u << 0) | (1u << 3) | (1u << 6) | (1u << 9) | (1u << 13);
+__device__ __forceinline__ int mhd_seq(int k) { return int((kMhdSymSeq >> (4 * k)) & 15ull); }
+__device__ __forceinline__ int mhd_z(int k) { return int((kMhdSymZ >> (4 * k)) & 15ull); }
+__device__ __forceinline__ int mhd_g(int k) { return int((kMhdSymG >> (4 * k)) & 15ull); }
+__device__ __forceinline__ int mhd_gform(int k) { return int((kMhdSymGForm >> (4 * k)) & 15ull); }
+
 // ----------------------------------------------------------------------------
 // retained-mode bookkeeping along one axis: positions n <= kp and
 // n >= N - kn are kept; compact index j = n (n <= kp) or n - (N - nr).
@@ -275,7 +303,13 @@ template <int MODE> struct XFwdShape;
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 4, NG = 3; };
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
-template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = 8, NG = 3; };
+// ideal MHD: the 15 distinct fluxes in three groups of 5 (mhd3_flux_sym; the
+// y pass CM_FWD_DIV3M applies i kx; PH_MHD_SYM 0: the 24 fluxes of the three
+// directions, i kx in the x pass)
+#ifndef PH_MHD_SYM
+#define PH_MHD_SYM 1
+#endif
+template <> struct XFwdShape<XF_MHD3>  { static constexpr int NC = 8, LX = PH_MHD_SYM ? 5 : 8, NG = 3; };
 template <> struct XFwdShape<XF_COPY8> { static constexpr int NC = 8, LX = 8, NG = 1; };
 
 // The Euler flux transforms of a row in one FFT over all NG*LX lines instead
@@ -312,8 +346,9 @@ enum ColMode {
   CM_FWDINV_CORR = 5, // forward, (sx sy sz - 1) on kept modes, inverse, full out
   CM_FWDINV_PROJ = 6, // three fields (B) together: forward, sigma and the Leray
                       // projection on kept modes, inverse, full out
-  CM_FWDINV_DIVACC = 7  // CM_FWDINV_DIV and the RK4 sum of the step's RHS spectra
+  CM_FWDINV_DIVACC = 7, // CM_FWDINV_DIV and the RK4 sum of the step's RHS spectra
                         // (below)
+  CM_FWD_DIV3M = 8      // 3D ideal-MHD y forward, symmetric flux layout: A(15) -> B(16)
 };
 
 template <typename T>
@@ -400,7 +435,8 @@ struct ColArgs {
 
 template <typename T, int N, int MODE>
 struct ColTile {
-  static constexpr bool PAIR = (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC);
+  static constexpr bool PAIR =
+      (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC || MODE == CM_FWD_DIV3M);
   static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
   static constexpr int ELEMS = MODE == CM_FWDINV_PROJ
                                    ? PH_COL_ELEMS_PROJ
@@ -475,22 +511,24 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   };
 
   // CM_FWD_DIV3, symmetric layout: kx of the tile's columns
-  __shared__ T s_colk[MODE == CM_FWD_DIV3 ? TM : 1];
-  if constexpr (MODE == CM_FWD_DIV3) {
-    if (a.symflux && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
+  constexpr bool SYMY = MODE == CM_FWD_DIV3 || MODE == CM_FWD_DIV3M;
+  __shared__ T s_colk[SYMY ? TM : 1];
+  if constexpr (SYMY) {
+    if ((MODE == CM_FWD_DIV3M || a.symflux) && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
   }
   constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : ((MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) ? 2 : 1);
   // this block's fields cb..ce-1 (all, or a.cpb of them by blockIdx.z:
   // launches of few tiles then have more, shorter blocks)
-  const int cb = (MODE != CM_FWD_DIV3 && a.cpb > 0) ? int(blockIdx.z) * a.cpb : 0;
-  const int ce = (MODE != CM_FWD_DIV3 && a.cpb > 0) ? (cb + a.cpb < nc ? cb + a.cpb : nc) : nc;
+  const int cb = (!SYMY && a.cpb > 0) ? int(blockIdx.z) * a.cpb : 0;
+  const int ce = (!SYMY && a.cpb > 0) ? (cb + a.cpb < nc ? cb + a.cpb : nc) : nc;
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
-    if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
+    if constexpr (MODE == CM_FWD_DIV3M) return mhd_seq(k);
+    else if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
     else if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) return (k % 2) * nc + cb + k / 2;
     else return cb + k;
   };
-  const int nsrc = (MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * (ce - cb);
+  const int nsrc = MODE == CM_FWD_DIV3M ? 15 : ((MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * (ce - cb));
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
     const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
@@ -552,6 +590,50 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           st(oc + (long long)j * ols + l, v);
         }
       });
+    } else if constexpr (MODE == CM_FWD_DIV3M) {
+      {
+        // ideal MHD, symmetric layout (nc = 8): the 15 distinct fluxes, each
+        // transformed once (order mhd_seq), feed G[c] -> field c and
+        // sigma F(Fz[c]) -> field 8 + c by the step tables above
+#pragma unroll 1
+        for (int s = 0; s < 15; ++s) {
+          fwd(FWD{});
+          const int zc = mhd_z(s), g1 = mhd_g(s), f1 = mhd_gform(s);
+          const bool zneg = (kMhdSymZNeg >> s) & 1u, g1neg = (kMhdSymGNeg >> s) & 1u;
+          const bool keep = (kMhdSymKeep >> s) & 1u, two = s == kMhdSymG2Step;
+          cx<T>* oz = out + (long long)(8 + zc) * a.out_fs + c0;
+          cx<T>* og1 = out + (long long)g1 * a.out_fs + c0;
+          cx<T>* og2 = out + (long long)kMhdSymG2 * a.out_fs + c0;
+          each([&](int q, int r, int l, int n, bool ok) {
+            const cx<T> x = X[q][r];
+            if (ok && kp.kept(n, N)) {
+              const int j = kp.compact(n, N);
+              const T sg = a.sig[j], kx = s_colk[l], ky = a.kk[j];
+              auto g_of = [&](int form) {
+                if (form == 3) return mul_ik(x, kx);
+                if (form == 4) return mul_ik(x, ky);
+                const cx<T> y0 = first(q, r, l, n);
+                return form == 1 ? mul_ik(y0, kx) + mul_ik(x, ky) : mul_ik(x, kx) + mul_ik(y0, ky);
+              };
+              if (zc < 15) st(oz + (long long)j * ols + l, cscale(zneg ? mk(-x.x, -x.y) : x, sg));
+              if (g1 < 15) {
+                const cx<T> v = g_of(f1);
+                st(og1 + (long long)j * ols + l, cscale(g1neg ? mk(-v.x, -v.y) : v, sg));
+              }
+              if (two) st(og2 + (long long)j * ols + l, cscale(g_of(kMhdSymG2Form), sg));
+            }
+            if (keep) {
+              if constexpr (YREG) Y[q][r] = x;
+              else if (kp.kept(n, N)) ybuf[n * TM + l] = x;
+            }
+          });
+        }
+        cx<T>* oz7 = out + (long long)15 * a.out_fs + c0;  // Fz[Bz] = 0
+        each([&](int q, int r, int l, int n, bool ok) {
+          if (ok && kp.kept(n, N)) st(oz7 + (long long)kp.compact(n, N) * ols + l, mk(T(0), T(0)));
+        });
+        break;
+      }
     } else if constexpr (MODE == CM_FWD_DIV3) {
       if (a.symflux) {
         // 3D Euler, symmetric layout (nc = 5; no i kx in the x pass): the 12
@@ -1194,6 +1276,34 @@ __device__ __forceinline__ T euler3_flux_sym(const T* w, int u) {
   return (u >= 3 && u <= 5) ? v + w[4] : v;
 }
 
+// The 15 distinct ideal-MHD fluxes from the primitives w = (rho, ux, uy, uz,
+// p*, Bx, By, Bz, E, u.B, E + p*) (phrike/equations.py:949-1000).  Of the 24
+// fluxes F_d[c], the momentum fluxes rho u_i u_j + p* delta_ij - B_i B_j are
+// symmetric in (i, j) and the induction fluxes u_d B_i - B_d u_i are
+// antisymmetric (zero for i = d):
+//   u = 0..2    rho ux, rho uy, rho uz                  (F_x,y,z[rho])
+//   u = 3..5    (rho ud) ud + p* - Bd Bd, d = x, y, z   (F_d[rho ud])
+//   u = 6..8    (rho ux) uy - Bx By, (rho ux) uz - Bx Bz, (rho uy) uz - By Bz
+//   u = 9..11   (E + p*) ud - Bd (u.B)                  (F_d[E])
+//   u = 12..14  ux By - uy Bx, ux Bz - uz Bx, uy Bz - uz By
+//               (F_x[By] = -F_y[Bx], F_x[Bz] = -F_z[Bx], F_y[Bz] = -F_z[By])
+// each in the reference's operation order for the first direction it names.
+template <typename T>
+__device__ __forceinline__ T mhd3_flux_sym(const T* w, int u) {
+  if (u < 3) return w[0] * w[1 + u];
+  if (u < 6) {
+    const int d = u - 3;
+    return (w[0] * w[1 + d] * w[1 + d] + w[4]) - w[5 + d] * w[5 + d];
+  }
+  if (u < 9) {
+    const int a = u == 8 ? 1 : 0, b = u == 6 ? 1 : 2;  // (x, y), (x, z), (y, z)
+    return (w[0] * w[1 + a]) * w[1 + b] - w[5 + a] * w[5 + b];
+  }
+  if (u < 12) return w[10] * w[1 + (u - 9)] - w[5 + (u - 9)] * w[9];
+  const int a = u == 14 ? 1 : 0, b = u == 12 ? 1 : 2;
+  return w[1 + a] * w[5 + b] - w[1 + b] * w[5 + a];
+}
+
 // Fluxes (or copies) of the conserved values q at the thread's x pairs ->
 // forward transform of the row -> kept modes to a.out.  Shared by k_xfwd and
 // k_xmid.
@@ -1279,9 +1389,11 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
   const int nl = xf_copy(MODE) ? ncomp : LX;
   // the 3D Euler groups unrolled: each line's flux is then fixed at compile
   // time (a run-time group index selects it through chains of selects)
-  constexpr int GU = (MODE == XF_FLUX3 && PH_XF_UNROLL_G) ? S::NG : 1;
-  // the 3D Euler fluxes are stored without i k (the y pass applies i kx)
-  constexpr bool IK = MODE != XF_FLUX3;
+  constexpr bool SYM = MODE == XF_FLUX3 || (MODE == XF_MHD3 && PH_MHD_SYM);
+  constexpr int GU = (SYM && PH_XF_UNROLL_G) ? S::NG : 1;
+  // the symmetric layouts' fluxes are stored without i k (the y pass applies
+  // i kx)
+  constexpr bool IK = !SYM;
 #pragma unroll GU
   for (int g = 0; g < S::NG; ++g) {
 #pragma unroll
@@ -1295,6 +1407,8 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
           for (int r = 0; r < 2; ++r) {
             if constexpr (xf_copy(MODE)) {
               v[r] = q[j][r][l];
+            } else if constexpr (MODE == XF_MHD3 && PH_MHD_SYM) {
+              v[r] = mhd3_flux_sym<T>(pr[j][r], g * LX + l);
             } else if constexpr (MODE == XF_MHD3) {
               // ideal-MHD flux of direction g, component l (phrike/equations.py:949-1000);
               // the direction is picked by selects (a register index by the

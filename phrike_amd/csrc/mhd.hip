@@ -162,7 +162,7 @@ struct PlanMHD3D : PlanBase {
 
   PlanMHD3D(int nx_, int ny_, int nz_, double lx, double ly, double lz, bool dealias, bool filt, int p,
             double alpha, double gamma_)
-      : main(nx_, ny_, nz_, lx, ly, lz, dealias, filt, p, alpha, gamma_, 24, 16),
+      : main(nx_, ny_, nz_, lx, ly, lz, dealias, filt, p, alpha, gamma_, PH_MHD_SYM ? 15 : 24, 16),
         prj(nx_, ny_, nz_, lx, ly, lz, false, false, p, alpha, gamma_, 3, 3),
         nx(nx_), ny(ny_), nz(nz_), gamma(gamma_) {
     dtype = sizeof(T) == 8 ? 0 : 1;

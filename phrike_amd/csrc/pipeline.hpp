@@ -377,6 +377,7 @@ struct Plan3D : PlanBase {
   T* unit_x = nullptr;     // ones by kept x mode (the unscaled forward transform)
   bool full = false;       // every mode kept and no filter: fft3d / ifft3d allowed
   bool symflux = true;     // Euler fluxes in the 12-field symmetric layout (x pass without i kx)
+  bool symmhd = false;     // ideal-MHD fluxes in the 15-field symmetric layout (x pass without i kx)
   // the Euler step keeps its RK4 sum in 3D spectral space (S, updated by the
   // z passes; the 2/3 rule on): no accumulator in the x kernels, and the
   // post-step filter is the stage-4 z pass's factor on S
@@ -392,7 +393,7 @@ struct Plan3D : PlanBase {
   Plan3D(int nx_, int ny_, int nz_, double lx_, double ly_, double lz_, bool dealias, bool filt,
          int p, double alpha, double gamma_, int na = 12, int nb = 10)
       : nx(nx_), ny(ny_), nz(nz_), lx(lx_), ly(ly_), lz(lz_), gamma(gamma_), symflux(na == 12),
-        specacc(na == 12 && dealias && PH_SPECACC != 0) {
+        symmhd(na == 15), specacc(na == 12 && dealias && PH_SPECACC != 0) {
     dtype = sizeof(T) == 8 ? 0 : 1;
     if (!supported_len(nx) || !supported_len(ny) || !supported_len(nz))
       throw Error(-2, "grid sizes must be powers of two in [8, 1024]");
@@ -561,9 +562,11 @@ struct Plan3D : PlanBase {
     a.kk = dy.k;
     a.symflux = div && symflux ? 1 : 0;
     if (a.symflux && nc != 5) throw Error(-2, "the symmetric flux layout takes the 5 Euler fields");
+    if (div && symmhd && nc != 8) throw Error(-2, "the symmetric MHD flux layout takes the 8 MHD fields");
     a.colk = xcol_k;
     KScope ks(div ? "yfwd_div" : (unit ? "yfwd_unit" : "yfwd_copy"), st);
-    if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
+    if (div && symmhd) launch_col<CE, CM_FWD_DIV3M>(ny, nz, a, st);
+    else if (div) launch_col<CE, CM_FWD_DIV3>(ny, nz, a, st);
     else launch_col<CE, CM_FWD_COPY>(ny, nz, a, st);
     check_launch("k_col(y fwd)");
   }
