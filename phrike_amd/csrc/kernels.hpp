@@ -6,10 +6,11 @@
 //            y-row (a half-length complex FFT), keeps |mx| <= Nx/3, scales by
 //            sigma_x/(Nx*Ny*Nz).  3D Euler: the 12 distinct fluxes (the
 //            momentum flux is symmetric), without i kx -> A[12][z][y][mx];
-//            2D Euler and MHD: every flux, x-fluxes times i kx
+//            3D MHD: the 15 distinct ones; 2D Euler: every flux, x-fluxes
+//            times i kx
 //   k_col    y pass: forward FFT along y of column tiles, keeps |my| <= Ny/3,
-//            forms G = sigma_y (i kx Fx^ + i ky Fy^) (2D / MHD: Fx^ already
-//            carries i kx) and sigma_y Fz^          -> B[10][z][my][mx]
+//            forms G = sigma_y (i kx Fx^ + i ky Fy^) (2D: Fx^ already carries
+//            i kx) and sigma_y Fz^                  -> B[10][z][my][mx]
 //   k_col    z pass: forward FFT along z of G and Fz^, D = sigma_z (G + i kz
 //            Fz^) on |mz| <= Nz/3, zero elsewhere (3D Euler steps: also the
 //            RK4 sum in spectral space, CM_FWDINV_DIVACC), inverse FFT along
