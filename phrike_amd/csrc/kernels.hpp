@@ -769,8 +769,7 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       lds_barrier();
       if constexpr (MODE == CM_FWDINV_DIVACC) {
         // the RK4 sum on the tile's kept modes, D read back from buf in
-        // whole tile rows (X is free here, so the S loads of a thread are
-        // issued together)
+        // whole tile rows (X is free here)
         const double dtd = a.dt_dev != nullptr ? a.dt_dev[0] : a.dt;
         const bool last = a.stage >= 4;
         const T w = T(dtd * ((a.stage == 1 || last) ? 1.0 / 6.0 : 1.0 / 3.0));
@@ -795,22 +794,22 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
           }
 #pragma unroll
           for (int ii = 0; ii < SB; ++ii) {
-          const int i = i0 + ii;
-          if (i >= NIT) break;
-          const int t = tid + i * NT;
-          const int j = t / TM, l = t - j * TM;
-          if (j < kp.nr && c0 + l < ncolw) {
-            const int n = kp.natural(j, N);
-            cx<T>* sp = sc + (long long)j * a.acc_ls + l;
-            const cx<T> s1 = s0[ii] - cscale(buf[n * TM + l], w);
-            if (last) {
-              const cx<T> u = cscale(s1, (a.colsig[c0 + l] * sy) * a.sig[j]);
-              st(sp, u);
-              buf[n * TM + l] = u;
-            } else {
-              st(sp, s1);
+            const int i = i0 + ii;
+            if (i >= NIT) break;
+            const int t = tid + i * NT;
+            const int j = t / TM, l = t - j * TM;
+            if (j < kp.nr && c0 + l < ncolw) {
+              const int n = kp.natural(j, N);
+              cx<T>* sp = sc + (long long)j * a.acc_ls + l;
+              const cx<T> s1 = s0[ii] - cscale(buf[n * TM + l], w);
+              if (last) {
+                const cx<T> u = cscale(s1, (a.colsig[c0 + l] * sy) * a.sig[j]);
+                st(sp, u);
+                buf[n * TM + l] = u;
+              } else {
+                st(sp, s1);
+              }
             }
-          }
           }
         }
         if (last) lds_barrier();
