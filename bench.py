@@ -553,12 +553,14 @@ def _ensemble_summary(res, wall, sizes):
                            "p99_s": round(float(np.percentile(lat, 99)), 3)}
     out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}
     out["per_size"] = per
-    # the 2D pencil-pipeline byte model C2(N) = 80 + 112 a per run-step (FP32)
-    # over the wall time, against 8 TB/s
+    # the 2D pencil-pipeline byte model C2(N) = 80 + 104 a words per cell and
+    # run-step (FP32; a = kept x columns as a fraction of N: the 7 fluxes
+    # written and read once per stage, the 4 y-inverted fields, the filter
+    # pass) over the wall time, against 8 TB/s
     moved = 0.0
     for r in res:
         a = 2 * (r.n // 3 + 1) / r.n
-        moved += r.steps * (80 + 112 * a) * 4 * r.n ** 2
+        moved += r.steps * (80 + 104 * a) * 4 * r.n ** 2
     out["model_GBs"] = round(moved / wall / 1e9, 1)
     out["hbm_frac_model"] = round(moved / wall / HBM_PEAK, 4)
     return out

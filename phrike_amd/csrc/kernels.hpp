@@ -6,11 +6,10 @@
 //            y-row (a half-length complex FFT), keeps |mx| <= Nx/3, scales by
 //            sigma_x/(Nx*Ny*Nz).  3D Euler: the 12 distinct fluxes (the
 //            momentum flux is symmetric), without i kx -> A[12][z][y][mx];
-//            3D MHD: the 15 distinct ones; 2D Euler: every flux, x-fluxes
-//            times i kx
+//            3D MHD: the 15 distinct ones; 2D Euler: the 7 distinct ones
 //   k_col    y pass: forward FFT along y of column tiles, keeps |my| <= Ny/3,
-//            forms G = sigma_y (i kx Fx^ + i ky Fy^) (2D: Fx^ already carries
-//            i kx) and sigma_y Fz^                  -> B[10][z][my][mx]
+//            forms G = sigma_y (i kx Fx^ + i ky Fy^) and sigma_y Fz^
+//                                                   -> B[10][z][my][mx]
 //   k_col    z pass: forward FFT along z of G and Fz^, D = sigma_z (G + i kz
 //            Fz^) on |mz| <= Nz/3, zero elsewhere (3D Euler steps: also the
 //            RK4 sum in spectral space, CM_FWDINV_DIVACC), inverse FFT along
@@ -302,6 +301,12 @@ template <int MODE> struct XFwdShape;
 // 3D Euler: the 12 distinct fluxes (the momentum flux tensor is symmetric),
 // 3 groups of 4 lines; see euler3_flux_sym
 template <> struct XFwdShape<XF_FLUX3> { static constexpr int NC = 5, LX = 4, NG = 3; };
+// 2D Euler: the 7 distinct fluxes (euler2_flux_sym) in groups of 4 and 3
+// lines, without i kx (the y pass CM_FWDINV_DIV2S applies it); PH_2D_SYM 0:
+// the 8 fluxes, x-fluxes times i kx
+#ifndef PH_2D_SYM
+#define PH_2D_SYM 1
+#endif
 template <> struct XFwdShape<XF_FLUX2> { static constexpr int NC = 4, LX = 4, NG = 2; };
 template <> struct XFwdShape<XF_COPY>  { static constexpr int NC = 5, LX = 5, NG = 1; };
 // ideal MHD: the 15 distinct fluxes in three groups of 5 (mhd3_flux_sym; the
@@ -349,7 +354,8 @@ enum ColMode {
                       // projection on kept modes, inverse, full out
   CM_FWDINV_DIVACC = 7, // CM_FWDINV_DIV and the RK4 sum of the step's RHS spectra
                         // (below)
-  CM_FWD_DIV3M = 8      // 3D ideal-MHD y forward, symmetric flux layout: A(15) -> B(16)
+  CM_FWD_DIV3M = 8,     // 3D ideal-MHD y forward, symmetric flux layout: A(15) -> B(16)
+  CM_FWDINV_DIV2S = 9   // 2D y pass, symmetric flux layout: the 7 fluxes -> 4 fields, in place
 };
 
 template <typename T>
@@ -437,7 +443,8 @@ struct ColArgs {
 template <typename T, int N, int MODE>
 struct ColTile {
   static constexpr bool PAIR =
-      (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC || MODE == CM_FWD_DIV3M);
+      (MODE == CM_FWD_DIV3 || MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC || MODE == CM_FWD_DIV3M ||
+       MODE == CM_FWDINV_DIV2S);
   static constexpr bool YREG = PAIR && ((sizeof(T) == 8) ? PH_COL_YREG_F64 : PH_COL_YREG_F32);
   static constexpr int ELEMS = MODE == CM_FWDINV_PROJ
                                    ? PH_COL_ELEMS_PROJ
@@ -512,10 +519,10 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   };
 
   // CM_FWD_DIV3, symmetric layout: kx of the tile's columns
-  constexpr bool SYMY = MODE == CM_FWD_DIV3 || MODE == CM_FWD_DIV3M;
+  constexpr bool SYMY = MODE == CM_FWD_DIV3 || MODE == CM_FWD_DIV3M || MODE == CM_FWDINV_DIV2S;
   __shared__ T s_colk[SYMY ? TM : 1];
   if constexpr (SYMY) {
-    if ((MODE == CM_FWD_DIV3M || a.symflux) && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
+    if ((MODE != CM_FWD_DIV3 || a.symflux) && tid < TM) s_colk[tid] = c0 + tid < ncolw ? a.colk[c0 + tid] : T(0);  // twiddle barrier below
   }
   constexpr int PERC = (MODE == CM_FWD_DIV3) ? 3 : ((MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) ? 2 : 1);
   // this block's fields cb..ce-1 (all, or a.cpb of them by blockIdx.z:
@@ -525,11 +532,13 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   // field read by the k-th global-source transform of this block
   auto field_of = [&](int k) {
     if constexpr (MODE == CM_FWD_DIV3M) return mhd_seq(k);
+    else if constexpr (MODE == CM_FWDINV_DIV2S) return int((0x6532410ull >> (4 * k)) & 15ull);  // 0 1 4 2 3 5 6
     else if constexpr (MODE == CM_FWD_DIV3) return a.symflux ? sym_seq(k) : (k % 3) * nc + k / 3;
     else if constexpr (MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC) return (k % 2) * nc + cb + k / 2;
     else return cb + k;
   };
-  const int nsrc = MODE == CM_FWD_DIV3M ? 15 : ((MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * (ce - cb));
+  const int nsrc = MODE == CM_FWD_DIV3M ? 15
+                   : (MODE == CM_FWDINV_DIV2S ? 7 : ((MODE == CM_FWD_DIV3 && a.symflux) ? 12 : PERC * (ce - cb)));
   // element (line l, position n) of field f's tile, zero outside the grid
   auto loader = [&](int k) {
     const cx<T>* src = in + (long long)field_of(k) * a.in_fs + c0;
@@ -741,7 +750,21 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
       // CM_FWDINV_DIV / CM_FWDINV_DIVACC / CM_FWDINV_COPY / CM_FWDINV_CORR:
       // forward, combine on kept modes, inverse
       constexpr bool DIV = MODE == CM_FWDINV_DIV || MODE == CM_FWDINV_DIVACC;
-      fwd(FWD{});
+      constexpr bool DIV2S = MODE == CM_FWDINV_DIV2S;
+      if constexpr (DIV2S) {
+        // fields 0 1 | 4 2 | 3 | 5 6 for components 0..3: each component but
+        // 2 keeps its first spectrum (rho ux, rho ux uy, (E+p) ux); component
+        // 2 transforms its y flux only and reuses rho ux uy, still kept, as
+        // its x flux.  The kept spectrum is component 1's y flux and the x
+        // flux of the others.
+        if (c != 2) {
+          fwd(FWD{});
+          keep_first();
+        }
+        fwd(FWD{});
+      } else {
+        fwd(FWD{});
+      }
       if constexpr (DIV) {
         keep_first();
         // CM_FWDINV_DIVACC loads the next field's first pass after the RK4
@@ -752,7 +775,11 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
         cx<T> v = mk(T(0), T(0));
         if (kp.kept(n, N)) {
           const int j = kp.compact(n, N);
-          if constexpr (DIV) {
+          if constexpr (DIV2S) {
+            const cx<T> kf = first(q, r, l, n), x = X[q][r];
+            const T kx = s_colk[l], ky = a.kk[j];
+            v = cscale(c == 1 ? mul_ik(x, kx) + mul_ik(kf, ky) : mul_ik(kf, kx) + mul_ik(x, ky), a.sig[j]);
+          } else if constexpr (DIV) {
             v = cscale(first(q, r, l, n) + mul_ik(X[q][r], a.kk[j]), a.sig[j]);
           } else if constexpr (MODE == CM_FWDINV_CORR) {
             // sx sy sz - 1 from the three (sigma - 1): no cancellation
@@ -1304,6 +1331,20 @@ __device__ __forceinline__ T mhd3_flux_sym(const T* w, int u) {
   return w[1 + a] * w[5 + b] - w[1 + b] * w[5 + a];
 }
 
+// The 7 distinct 2D Euler fluxes from the primitives w = (rho, ux, uy, 0, p,
+// E + p) (phrike/equations.py:119-134): u = 0, 1: rho ux, rho uy; 2, 3:
+// (rho ux) ux + p, (rho uy) uy + p; 4: (rho ux) uy (F_x[rho v] = F_y[rho u];
+// the reference forms the latter as (rho uy) ux, one rounding apart at most);
+// 5, 6: (E + p) ux, (E + p) uy; 7: none (zero)
+template <typename T>
+__device__ __forceinline__ T euler2_flux_sym(const T* w, int u) {
+  if (u < 2) return w[0] * w[1 + u];
+  if (u < 4) return (w[0] * w[u - 1]) * w[u - 1] + w[4];
+  if (u == 4) return (w[0] * w[1]) * w[2];
+  if (u < 7) return w[5] * w[u - 4];
+  return T(0);
+}
+
 // Fluxes (or copies) of the conserved values q at the thread's x pairs ->
 // forward transform of the row -> kept modes to a.out.  Shared by k_xfwd and
 // k_xmid.
@@ -1365,6 +1406,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
   }
   cx<T>* dst = a.out + (long long)slab * a.out_ss + (long long)y * a.mxp;
   if constexpr (XMerge<T, NX, MODE>::value) {
+    static_assert(!(MODE == XF_FLUX2 && PH_2D_SYM), "the merged transform takes the general 2D layout");
     // all NG directions' fluxes as one batch of lines (group g = lines g*LX..)
     constexpr int LG = XMerge<T, NX, MODE>::lines;
 #pragma unroll
@@ -1386,10 +1428,11 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
     lds_barrier();
     return;
   }
-  const int nl = xf_copy(MODE) ? ncomp : LX;
+  const int nl0 = xf_copy(MODE) ? ncomp : LX;
   // the 3D Euler groups unrolled: each line's flux is then fixed at compile
   // time (a run-time group index selects it through chains of selects)
-  constexpr bool SYM = MODE == XF_FLUX3 || (MODE == XF_MHD3 && PH_MHD_SYM);
+  constexpr bool SYM2 = MODE == XF_FLUX2 && PH_2D_SYM;
+  constexpr bool SYM = MODE == XF_FLUX3 || (MODE == XF_MHD3 && PH_MHD_SYM) || SYM2;
   constexpr int GU = (SYM && PH_XF_UNROLL_G) ? S::NG : 1;
   // the symmetric layouts' fluxes are stored without i k (the y pass applies
   // i kx)
@@ -1427,7 +1470,7 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
             } else if constexpr (MODE == XF_FLUX3) {
               v[r] = euler3_flux_sym<T>(pr[j][r], g * LX + l);
             } else {
-              v[r] = euler_flux<T, NC>(pr[j][r], g, l);
+              v[r] = SYM2 ? euler2_flux_sym<T>(pr[j][r], g * LX + l) : euler_flux<T, NC>(pr[j][r], g, l);
             }
           }
           buf[xslot<LX, T>(p, l)] = mk(v[0], v[1]);
@@ -1436,6 +1479,8 @@ __device__ __forceinline__ void xr_forward(const XFwdArgs<T>& a, int ncomp, cons
     }
     lds_barrier();
     fft_lds<T, M, LX, -1, NT, xstride(LX, sizeof(cx<T>)), xpadb(LX, sizeof(cx<T>)), XPlan<M>, 2, PH_XTW_SWZ>(buf, tw, tid);
+    // the 2D symmetric layout's second group holds 3 fluxes
+    const int nl = (SYM2 && g == S::NG - 1) ? 3 : nl0;
     xr_split_store<T, NX, DA, LX, LX, IK>(buf, tw, dst + (long long)(g * LX) * a.out_fs, a.out_fs, nl, sig, kk,
                                           (a.ikmask >> (g * LX)) & 1u, tid);
     lds_barrier();

@@ -1046,6 +1046,7 @@ struct Plan2D : PlanBase {
   DevAxis<CE> dy;
   int mx, mxp;
   long long ncell;
+  CE* xcol_k = nullptr;  // kx per column of the y pass (zero past the kept modes)
   size_t offA, offUs, offPart, offCfl, wsBytes;
   static constexpr int NPART = 256;
 
@@ -1064,6 +1065,19 @@ struct Plan2D : PlanBase {
     ncell = (long long)nx * ny;
     dx.upload(ax, 1.0 / (double(nx) * double(ny)));
     dy.upload(ay, 1.0);
+    {
+      const int ncol = (mx + PW - 1) / PW, ncolw = mxp / PW;
+      std::vector<CE> hk(ncolw, CE(0.0));
+      for (int c = 0; c < ncol; ++c) {
+        if constexpr (PW == 2) {
+          hk[c] = CE(float(ax.k[2 * c]), float(2 * c + 1 < mx ? ax.k[2 * c + 1] : 0.0));
+        } else {
+          hk[c] = CE(ax.k[c]);
+        }
+      }
+      hip_check(hipMalloc(&xcol_k, sizeof(CE) * ncolw), "hipMalloc(column k)");
+      hip_check(hipMemcpy(xcol_k, hk.data(), sizeof(CE) * ncolw, hipMemcpyHostToDevice), "copy column k");
+    }
     const size_t cs = sizeof(cx<T>);
     auto al = [](size_t v) { return (v + 255) / 256 * 256; };
     offA = 0;
@@ -1073,6 +1087,7 @@ struct Plan2D : PlanBase {
     wsBytes = al(offCfl + sizeof(double) * (size_t)batch * ny);
   }
   ~Plan2D() override {
+    if (xcol_k) (void)hipFree(xcol_k);
     dx.release();
     dy.release();
   }
@@ -1146,7 +1161,10 @@ struct Plan2D : PlanBase {
     a.tw = dy.tw;
     a.sig = dy.sig;
     a.kk = dy.k;
-    if (div) launch_col<CE, CM_FWDINV_DIV>(ny, nb, a, st);
+    a.colk = xcol_k;
+    // the symmetric 2D layout (7 fluxes, no i kx in the x pass)
+    if (div && PH_2D_SYM) launch_col<CE, CM_FWDINV_DIV2S>(ny, nb, a, st);
+    else if (div) launch_col<CE, CM_FWDINV_DIV>(ny, nb, a, st);
     else launch_col<CE, CM_FWDINV_COPY>(ny, nb, a, st);
     check_launch("k_col(2d y)");
   }
