@@ -559,10 +559,76 @@ def _ensemble_summary(res, wall, sizes):
     # pass) over the wall time, against 8 TB/s
     moved = 0.0
     for r in res:
-        a = 2 * (r.n // 3 + 1) / r.n
-        moved += r.steps * (80 + 104 * a) * 4 * r.n ** 2
+        moved += r.steps * khi2d_words(r.n) * 4 * r.n ** 2
     out["model_GBs"] = round(moved / wall / 1e9, 1)
     out["hbm_frac_model"] = round(moved / wall / HBM_PEAK, 4)
+    return out
+
+
+def khi2d_words(n: int) -> float:
+    """Words moved per cell and run-step by the chained batched 2D step (the
+    pencil-pipeline byte model C2(N) = 80 + 104 a, a = kept x columns as a
+    fraction of N)."""
+    a = 2 * (n // 3 + 1) / n
+    return 80 + 104 * a
+
+
+def khi2d_batch_study(dev, cases=((1024, 32), (512, 128), (256, 256), (128, 512)), precisions=("single", "double"),
+                      steps: int = 32, launches: int = 3):
+    """The batched 2D step at fixed batch sizes, as the ensemble server runs
+    it: periodic KHI runs (seeds 0..B-1, filter every step, CFL 0.4), one
+    ph_run2d call advancing every run by `steps` chained steps with its dt
+    from the CFL maxima the previous step left (no host round trip), timed
+    over `launches` calls after one warm-up call.  Reports ms per batched
+    step, Mcell-updates/s and the byte model C2(N) against 8 TB/s."""
+    from phrike_amd import _lib
+    from phrike_amd.equations import EulerEquations2D
+    from phrike_amd.grid import Grid2D
+    from phrike_amd.initial_conditions import khi_2d_periodic
+    from phrike_amd.solver import SpectralSolver2D
+    import torch
+    fp = {"enabled": True, "p": 16, "alpha": 36.0, "interval": 1}
+    out = {"workload": f"KHI 2D periodic, B runs of N^2 per batch, filter every step, CFL 0.4, {steps} chained "
+                       f"steps per ph_run2d call, {launches} timed calls after 1 warm-up"}
+    for prec in precisions:
+        es = 4 if prec == "single" else 8
+        for n, nb in cases:
+            print(f"[bench] khi2d batch {n}^2 x {nb} {prec}", file=sys.stderr, flush=True)
+            g = Grid2D(n, n, 1.0, 1.0, dealias=True, filter_params=fp, backend="hip", precision=prec, batch=nb,
+                       torch_device=str(dev))
+            s = SpectralSolver2D(g, EulerEquations2D(1.4), cfl=0.4)
+            U0 = np.stack([khi_2d_periodic(n, n, seed=i) for i in range(min(nb, 4))])
+            U = [torch.as_tensor(U0, dtype=g.dtype, device=dev).repeat((nb + 3) // 4, 1, 1, 1)[:nb].contiguous(),
+                 torch.empty((nb, 4, n, n), dtype=g.dtype, device=dev)]
+            rs = torch.zeros(_lib.PH_RS_N * nb, dtype=torch.float64, device=dev)
+            rf = torch.zeros(_lib.PH_RF_N * nb, dtype=torch.int32, device=dev)
+            rs[_lib.PH_RS_TEND * nb:(_lib.PH_RS_TEND + 1) * nb] = 1e30
+            rs[_lib.PH_RS_TSTOP * nb:(_lib.PH_RS_TSTOP + 1) * nb] = float("inf")
+            rf[_lib.PH_RF_ACTIVE * nb:(_lib.PH_RF_ACTIVE + 1) * nb] = 1
+            plan, ws, st = s._ctx()
+            _lib.call("ph_max_wave_speed2d", plan, nb, _lib.ptr(U[0]), None,
+                      _lib.ptr(rs[_lib.PH_RS_DT * nb:]), 0.4, None, ws, st)
+
+            def launch():
+                _lib.call("ph_run2d", plan, nb, _lib.ptr(U[0]), _lib.ptr(U[1]), steps, _lib.ptr(rs), _lib.ptr(rf),
+                          0.4, ws, st)
+
+            launch()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                launch()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / (launches * steps)
+            cells = nb * n * n
+            gbs = khi2d_words(n) * es * cells / (ms * 1e-3) / 1e9
+            done = int(rf[_lib.PH_RF_STEPS * nb:(_lib.PH_RF_STEPS + 1) * nb].min())
+            finite = bool(torch.isfinite(U[0]).all())
+            out[f"{n}x{nb}_{prec}"] = {"ms_per_step": round(ms, 4), "mcell_updates_per_s": round(cells / ms / 1e3, 1),
+                                       "model_GBs": round(gbs, 1), "hbm_frac_model": round(gbs * 1e9 / HBM_PEAK, 4),
+                                       "steps_done_min": done, "finite": finite}
+            del g, s, U, rs, rf
+            torch.cuda.empty_cache()
     return out
 
 
@@ -766,6 +832,8 @@ def main():
     ap.add_argument("--mhd-only", action="store_true", help="only run the ideal-MHD study")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the ensemble serving study (stated stream, saturation, sustainable rate)")
+    ap.add_argument("--khi2d-only", action="store_true",
+                    help="only run the batched 2D step study (fixed batches, ph_run2d)")
     ap.add_argument("--gll-only", action="store_true",
                     help="only run the batched D.U matrix-core study (for profiling the MFMA kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -815,6 +883,9 @@ def main():
         return
     if args.mhd_only:
         print(json.dumps({"mhd3d": mhd_study(args.mhd_size, args.mhd_steps, torch.device("cuda:0"))}))
+        return
+    if args.khi2d_only:
+        print(json.dumps({"khi2d_batch": khi2d_batch_study(torch.device("cuda:0"))}))
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_study(args.ensemble_rate, args.ensemble_window,
@@ -948,6 +1019,8 @@ def main():
             print(f"[bench] ideal MHD {args.mhd_size}^3", file=sys.stderr, flush=True)
             extras["mhd3d"] = mhd_study(args.mhd_size, args.mhd_steps, dev)
             torch.cuda.empty_cache()
+        extras["khi2d_batch"] = khi2d_batch_study(dev)
+        torch.cuda.empty_cache()
         if args.ensemble_window > 0:
             print(f"[bench] ensemble: {args.ensemble_rate} req/s over {args.ensemble_window} s",
                   file=sys.stderr, flush=True)
