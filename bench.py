@@ -553,10 +553,8 @@ def _ensemble_summary(res, wall, sizes):
                            "p99_s": round(float(np.percentile(lat, 99)), 3)}
     out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}
     out["per_size"] = per
-    # the 2D pencil-pipeline byte model C2(N) = 80 + 104 a words per cell and
-    # run-step (FP32; a = kept x columns as a fraction of N: the 7 fluxes
-    # written and read once per stage, the 4 y-inverted fields, the filter
-    # pass) over the wall time, against 8 TB/s
+    # the 2D pencil-pipeline byte model (khi2d_words, FP32) over the wall
+    # time, against 8 TB/s
     moved = 0.0
     for r in res:
         moved += r.steps * khi2d_words(r.n) * 4 * r.n ** 2
@@ -566,11 +564,18 @@ def _ensemble_summary(res, wall, sizes):
 
 
 def khi2d_words(n: int) -> float:
-    """Words moved per cell and run-step by the chained batched 2D step (the
-    pencil-pipeline byte model C2(N) = 80 + 104 a, a = kept x columns as a
-    fraction of N)."""
+    """Words moved per cell and run-step by the chained batched 2D step with
+    the filter every step, C2(N) = 40 + 104 a (a = kept x columns as a
+    fraction of N).  Real fields: stage 1 reads U and writes the RK4
+    accumulator (8), stages 2-3 read U and the accumulator and write it (12
+    each), stage 4 reads the accumulator (4), the last x pass writes the new
+    state (4).  Kept-column spectra: per stage the y pass reads the 7 fluxes
+    and writes 4 fields (11 a), stages 1-3 read 4 and write 7 (11 a each),
+    stage 4 reads 4 and writes the state's 4 for the filter (8 a), the filter's
+    y pass reads and writes 4 (8 a), the last x pass reads 4 and writes the
+    next step's 7 fluxes (11 a)."""
     a = 2 * (n // 3 + 1) / n
-    return 80 + 104 * a
+    return 40 + 104 * a
 
 
 def khi2d_batch_study(dev, cases=((1024, 32), (512, 128), (256, 256), (128, 512)), precisions=("single", "double"),
