@@ -466,6 +466,28 @@ struct ColTile {
 #define PH_COL_WAVES_PER_EU 2
 #endif
 
+// Block -> tile.  Tiles narrower than a 128-byte line (TM < 8 at long lines)
+// share each line with the next tile, and blocks go round-robin over the 8
+// XCDs, each with its own L2: in x order the two tiles of a line sit on two
+// XCDs and the line is fetched from HBM twice.  Within each run of 16 blocks
+// of a row, blocks i and i + 8 (one XCD, dispatched together) take tiles 2i
+// and 2i + 1; a shorter last run pairs what it can and keeps the rest in
+// order (a permutation of the row's tiles).  PH_COL_XCD_PAIR_ALL applies it
+// at every tile width (tests of the mapping).
+#ifndef PH_COL_XCD_PAIR
+#define PH_COL_XCD_PAIR 1
+#endif
+#ifndef PH_COL_XCD_PAIR_ALL
+#define PH_COL_XCD_PAIR_ALL 0
+#endif
+__device__ __forceinline__ int xcd_pair_tile(int b, int g) {
+  const int base = b & ~15, t = g - base < 16 ? g - base : 16, i = b - base;
+  if (t <= 8) return b;
+  if (i < t - 8) return base + 2 * i;
+  if (i >= 8) return base + 2 * (i - 8) + 1;
+  return base + 2 * (t - 8) + (i - (t - 8));
+}
+
 // Per block, the global-source transforms run in a fixed order (field
 // sequence below); the pass-0 inputs of transform k+1 are loaded while
 // transform k runs its LDS passes.  For the two-input modes the first
@@ -483,7 +505,8 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   __shared__ cx<T> buf[N * TM];
   if (gated_off(a.active, a.act_all, blockIdx.y)) return;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * TM;
+  constexpr bool XPAIR = PH_COL_XCD_PAIR_ALL || (PH_COL_XCD_PAIR && TM * sizeof(cx<T>) < 128);
+  const int c0 = (XPAIR ? xcd_pair_tile(int(blockIdx.x), int(gridDim.x)) : int(blockIdx.x)) * TM;
   for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
   const cx<T>* in = a.in + (long long)blockIdx.y * a.in_bs;
   cx<T>* out = a.out + (long long)blockIdx.y * a.out_bs;
@@ -494,7 +517,8 @@ __global__ __launch_bounds__((ColTile<T, N, MODE>::NT), PH_COL_WAVES_PER_EU) voi
   const int nc = a.nc;
   // tile rows of a whole 128-byte line stream (non-temporal); narrower rows
   // (long lines: TM < 8) keep cached accesses, so the two halves of a line,
-  // read and written by neighbouring tiles, meet in L2 rather than in HBM
+  // read and written by the tiles of one XCD (xcd_pair_tile), meet in L2
+  // rather than in HBM
   constexpr bool LINE = TM * sizeof(cx<T>) >= 128;
   auto ld = [](const cx<T>* p) {
     if constexpr (LINE) return ld_stream(p);
