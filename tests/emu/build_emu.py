@@ -16,7 +16,9 @@ LIB = os.path.join(OUT, "libphrike_emu.so")
 SOURCES = ["capi.hip", "inst_f64.hip", "inst_f32.hip", "hybrid.hip", "mhd.hip", "emu_stubs.cpp"]
 
 
-def build(asan: bool = False) -> str:
+def build(asan: bool = False, defines=(), tag: str = "") -> str:
+    """The emulation library; `defines` (kernel compile-time options) go to
+    a library of its own, named by `tag`."""
     # objects in a per-process folder and an atomic rename of the library, so
     # parallel test workers building at once never load a half-written file
     work = os.path.join(OUT, f"p{os.getpid()}")
@@ -24,6 +26,7 @@ def build(asan: bool = False) -> str:
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-I", HERE, "-Wno-unused-result"]
     # extra defines for tuning variants of the kernels, e.g. EMU_DEFINES="-DPH_XPADB=4"
     flags += os.environ.get("EMU_DEFINES", "").split()
+    flags += list(defines)
     if asan:
         flags += ["-fsanitize=address", "-fno-omit-frame-pointer"]
     objs = []
@@ -39,6 +42,8 @@ def build(asan: bool = False) -> str:
         if p.returncode != 0:
             raise RuntimeError(out)
     lib = LIB.replace(".so", "_asan.so") if asan else LIB
+    if tag:
+        lib = lib.replace(".so", f"_{tag}.so")
     tmp = os.path.join(work, os.path.basename(lib))
     r = subprocess.run(["g++", *flags, "-shared", "-o", tmp, *objs, "-lpthread"], capture_output=True, text=True)
     if r.returncode != 0:

@@ -10,9 +10,9 @@ from phrike_amd import _lib as _real
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def load(asan=False):
+def load(asan=False, defines=(), tag=""):
     from build_emu import build
-    path = build(asan)
+    path = build(asan, defines, tag)
     lib = ctypes.CDLL(path)
     for name, (res, args) in _real._SIGNATURES.items():
         if name.startswith("ph_gll"):

@@ -481,3 +481,23 @@ def test_run2d_chained_steps(emu):
     np.testing.assert_array_equal(rs, rs1)
     np.testing.assert_array_equal(U, U1)
     np.testing.assert_array_equal(V, V1)
+
+
+def test_column_tile_pairing_is_a_permutation(emu):
+    """The XCD pairing of column tiles (xcd_pair_tile: blocks i and i + 8 of
+    each run of 16 take tiles 2i and 2i + 1) forced at every tile width, with
+    2-column tiles so that a row of the grid holds more than 8 tiles (12 at
+    the sizes here): only the block-to-tile assignment changes, so 3D steps
+    and batched 2D steps, FP64 and FP32, give the default build's bits."""
+    from emu_lib import Emu2D, Emu3D, load
+    from phrike_amd.initial_conditions import khi_2d_periodic, taylor_green_3d
+    paired = load(defines=("-DPH_COL_XCD_PAIR_ALL=1", "-DPH_COL_TM_MAX=2"), tag="pair")
+    rng = np.random.default_rng(3)
+    # kept x columns 22 (FP64, row pitch 24) and 43 (FP32, 24 packed pairs)
+    for dtype, nx in ((np.float64, 64), (np.float32, 128)):
+        U = (taylor_green_3d(nx, 16, 8) + 1e-3 * rng.standard_normal((5, 8, 16, nx))).astype(dtype)
+        a, b = Emu3D(emu, nx, 16, 8, dtype=dtype), Emu3D(paired, nx, 16, 8, dtype=dtype)
+        np.testing.assert_array_equal(a.step(U, 1e-3), b.step(U, 1e-3))
+        W = np.ascontiguousarray(np.stack([khi_2d_periodic(nx, 32, seed=s) for s in range(3)]).astype(dtype))
+        a2, b2 = Emu2D(emu, nx, 32, batch=3, dtype=dtype), Emu2D(paired, nx, 32, batch=3, dtype=dtype)
+        np.testing.assert_array_equal(a2.step(W, 1e-3), b2.step(W, 1e-3))
